@@ -1,0 +1,135 @@
+/*
+ * pt_abi.h — C ABI of the MI355X (gfx950) PandemicThrust daily-loop engine.
+ *
+ * The reference has no plugin or FFI interface: its daily loop sits behind
+ * the PandemicSim class (PandemicSim.h:68-281) whose device state lives in
+ * process-global __constant__ symbols (PandemicSim.cu:26-96).  This header
+ * replaces that kernel-level boundary and the device half of the class with
+ * plain C entry points: one handle per simulation, no global device state,
+ * integer return codes (0 ok, negative error; the library never exits), and
+ * caller-owned host buffers.  A handle binds one HIP stream and is not
+ * thread-safe; several handles may coexist in one process.
+ *
+ * Citations are relative to PandemicThrust/PandemicThrust/ of the reference.
+ */
+#ifndef PANDEMIC_PT_ABI_H
+#define PANDEMIC_PT_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_OK 0
+#define PT_EINVAL (-1)   /* bad argument or input outside the model's range */
+#define PT_ENOMEM (-2)   /* device allocation failed */
+#define PT_EDEVICE (-3)  /* HIP runtime error */
+#define PT_ESTATE (-4)   /* call out of order (e.g. step before setup) */
+
+/* The data row of constants.csv in file order (PandemicSim.cu:299-309). */
+typedef struct pt_params {
+    int32_t max_days;            /* 1..255 (day_t is 8 bits, PandemicSim.h:76) */
+    float r_pandemic;
+    float r_seasonal;
+    int32_t init_pandemic;
+    int32_t init_seasonal;
+    float people_scaling;        /* households = round(people_scaling * 1e6) */
+    float location_scaling;      /* per-type workplace counts scaled by this */
+    float percent_symptomatic;
+    float asymp_factor;
+} pt_params;
+
+typedef struct pt_sim pt_sim;
+
+/* Text of the last error raised on the calling thread. */
+const char* pt_last_error(void);
+
+/* glibc seed sequence of setup_loadSeed (PandemicSim.cu:2843-2862):
+ * srand(core_seed); seeds[i] = rand() for i < 4.  Host only. */
+int pt_glibc_seeds(int32_t core_seed, int32_t seeds_out[4]);
+
+/* Initial infected of setup_initialInfected (PandemicSim.cu:708-719) via
+ * n_unique_numbers (:1642-1659): srand(core_seed), skip the 4 seed draws,
+ * then n unique rand() % n_people.  Host only. */
+int pt_glibc_initial_infected(int32_t core_seed, int32_t n_people, int32_t n, int32_t* ids_out);
+
+/* PandemicSim() constructor device half (PandemicSim.cu:98-133):
+ * setup_scaleSimulation (:1127), setup_calculateInfectionData (:2828) and
+ * setup_pushDeviceData (:569).  seed_key = SEED[0], SEED[1] (the Threefry
+ * key).  device = HIP device ordinal. */
+int pt_sim_create(const pt_params* params, const int32_t seed_key[2], int32_t device, pt_sim** out);
+
+/* setupSim (PandemicSim.cu:146-208) up to, not including, the initial
+ * infections: population size, households, workplaces, workplace table and
+ * status initialisation (K1-K7).  Returns the population size. */
+int pt_sim_setup(pt_sim* sim, int64_t* n_people_out);
+
+/* setup_initialInfected (PandemicSim.cu:708-758): the first n_pandemic ids
+ * get the pandemic strain, the next n_seasonal the seasonal strain
+ * (status 2, day 0, generation 0).  Ids must be unique and < n_people. */
+int pt_sim_seed_infections(pt_sim* sim, const int32_t* ids, int32_t n_pandemic, int32_t n_seasonal);
+
+/* One iteration of runToCompletion (PandemicSim.cu:882-954): count and
+ * recover, compact, then the weekday or weekend step.  Writes the day's 16
+ * status counts (daily_writeInfectedStats order: pandemic codes 0..7, then
+ * seasonal 0..7) and the infected count.  Returns 0 after a day with
+ * contacts, 1 when the loop has ended (I == 0 day or max_days reached; the
+ * I == 0 day still reports its counts), negative on error. */
+int pt_sim_step(pt_sim* sim, int32_t counts16_out[16], int32_t* infected_out);
+
+/* Runs the remaining days of runToCompletion without per-day host
+ * synchronisation.  counts_out[days*16], infected_out[days] (either may be
+ * NULL); *days_out = number of rows (days whose counts were produced). */
+int pt_sim_run(pt_sim* sim, int32_t* counts_out, int32_t* infected_out, int32_t* days_out);
+
+/* final_countReproduction (PandemicSim.cu:2227-2304) in histogram form:
+ * c[k] = number of agents whose generation is < k for k < max_days, and
+ * c[max_days] = n_people.  len must be >= max_days + 1. */
+int pt_sim_generation_offsets(pt_sim* sim, int32_t* c_pandemic, int32_t* c_seasonal, int32_t len);
+
+/* Parity hooks: copy a device array into a host buffer. */
+enum pt_array {
+    PT_STATUS_P = 0, PT_STATUS_S, PT_DAY_P, PT_DAY_S, PT_GEN_P, PT_GEN_S, /* uint8[N] */
+    PT_AGE,             /* uint8[N] */
+    PT_HOUSEHOLD,       /* int32[N] people_households */
+    PT_HH_OFFSETS,      /* int32[H+1] household_offsets */
+    PT_WP_OFFSETS,      /* int32[W+1] workplace_offsets */
+    PT_WP_PEOPLE,       /* int32[N] workplace_people */
+    PT_INFECTED,        /* int32[I] infected_indexes of the last day run */
+    PT_ERRAND_OFFSETS,  /* int32[hours*W+1] errand_locationOffsets of the last day, reference layout */
+    PT_ERRAND_PEOPLE,   /* int32[E*N] sorted errand table of the last day */
+    PT_HH_TYPES         /* uint8[H+1] household types */
+};
+int64_t pt_sim_array_bytes(pt_sim* sim, int32_t which);
+int pt_sim_copy_array(pt_sim* sim, int32_t which, void* dst, int64_t bytes);
+
+/* Scalars: [0] N, [1] H, [2] W, [3] rand_offset, [4] workplace rand offset,
+ * [5] errand rand offset of the last day, [6] current day, [7] days run. */
+int pt_sim_get_scalars(pt_sim* sim, int64_t out[8]);
+
+/* Saves the post-setup, post-seeding state on the device; restore brings
+ * the simulation back to it (used to repeat the daily loop). */
+int pt_sim_snapshot(pt_sim* sim);
+int pt_sim_restore(pt_sim* sim);
+
+/* Per-kernel-class timing with HIP events on the handle's stream.
+ * enable != 0 starts recording (clears totals).  pt_sim_kernel_times
+ * fills ms_out[k] (total milliseconds) and launches_out[k] for k < n and
+ * returns the number of classes; pt_sim_kernel_name names class k. */
+int pt_sim_enable_timing(pt_sim* sim, int32_t enable);
+int pt_sim_kernel_times(pt_sim* sim, double* ms_out, int64_t* launches_out, int32_t n);
+const char* pt_sim_kernel_name(int32_t k);
+
+/* Blocks until the handle's stream is idle; loop time of the last
+ * pt_sim_run in milliseconds (HIP events around the enqueued days). */
+int pt_sim_synchronize(pt_sim* sim);
+double pt_sim_last_run_ms(pt_sim* sim);
+
+void pt_sim_destroy(pt_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PANDEMIC_PT_ABI_H */

@@ -1,0 +1,199 @@
+"""ctypes binding of the gfx950 library (lib/libpt_hip.so, C ABI in
+include/pandemic/pt_abi.h).
+
+PandemicSim mirrors the reference's host flow (PandemicThrust/PandemicThrust/
+main.cu:12-31): construct, setup_sim(), run_to_completion().  Every call goes
+to the HIP library; there is no CPU fallback, and importing raises when the
+library is not built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libpt_hip.so")
+
+PT_STATUS_P, PT_STATUS_S, PT_DAY_P, PT_DAY_S, PT_GEN_P, PT_GEN_S = range(6)
+PT_AGE, PT_HOUSEHOLD, PT_HH_OFFSETS, PT_WP_OFFSETS, PT_WP_PEOPLE = range(6, 11)
+PT_INFECTED, PT_ERRAND_OFFSETS, PT_ERRAND_PEOPLE, PT_HH_TYPES = range(11, 15)
+_U8_ARRAYS = {PT_STATUS_P, PT_STATUS_S, PT_DAY_P, PT_DAY_S, PT_GEN_P, PT_GEN_S, PT_AGE, PT_HH_TYPES}
+
+EXPORTS = [
+    "pt_last_error", "pt_glibc_seeds", "pt_glibc_initial_infected", "pt_sim_create", "pt_sim_setup",
+    "pt_sim_seed_infections", "pt_sim_step", "pt_sim_run", "pt_sim_generation_offsets", "pt_sim_array_bytes",
+    "pt_sim_copy_array", "pt_sim_get_scalars", "pt_sim_snapshot", "pt_sim_restore", "pt_sim_enable_timing",
+    "pt_sim_kernel_times", "pt_sim_kernel_name", "pt_sim_synchronize", "pt_sim_last_run_ms", "pt_sim_destroy",
+]
+
+
+class Params(C.Structure):
+    """The data row of constants.csv (PandemicSim.cu:299-309)."""
+    _fields_ = [
+        ("max_days", C.c_int32),
+        ("r_pandemic", C.c_float),
+        ("r_seasonal", C.c_float),
+        ("init_pandemic", C.c_int32),
+        ("init_seasonal", C.c_int32),
+        ("people_scaling", C.c_float),
+        ("location_scaling", C.c_float),
+        ("percent_symptomatic", C.c_float),
+        ("asymp_factor", C.c_float),
+    ]
+
+    @classmethod
+    def from_row(cls, fields):
+        f = [x.strip() for x in fields]
+        return cls(int(f[0]), float(f[1]), float(f[2]), int(f[3]), int(f[4]), float(f[5]), float(f[6]),
+                   float(f[7]), float(f[8]))
+
+
+class PTError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Loads the HIP library (raises ImportError if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libpt_hip.so is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.pt_last_error.restype = C.c_char_p
+        L.pt_glibc_seeds.argtypes = [C.c_int32, C.c_void_p]
+        L.pt_glibc_initial_infected.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
+        L.pt_sim_create.argtypes = [C.POINTER(Params), C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]
+        L.pt_sim_setup.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        L.pt_sim_seed_infections.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]
+        L.pt_sim_step.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]
+        L.pt_sim_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]
+        L.pt_sim_generation_offsets.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
+        L.pt_sim_array_bytes.restype = C.c_int64
+        L.pt_sim_array_bytes.argtypes = [C.c_void_p, C.c_int32]
+        L.pt_sim_copy_array.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]
+        L.pt_sim_get_scalars.argtypes = [C.c_void_p, C.c_void_p]
+        L.pt_sim_snapshot.argtypes = [C.c_void_p]
+        L.pt_sim_restore.argtypes = [C.c_void_p]
+        L.pt_sim_enable_timing.argtypes = [C.c_void_p, C.c_int32]
+        L.pt_sim_kernel_times.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
+        L.pt_sim_kernel_name.restype = C.c_char_p
+        L.pt_sim_kernel_name.argtypes = [C.c_int32]
+        L.pt_sim_synchronize.argtypes = [C.c_void_p]
+        L.pt_sim_last_run_ms.restype = C.c_double
+        L.pt_sim_last_run_ms.argtypes = [C.c_void_p]
+        L.pt_sim_destroy.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise PTError(lib().pt_last_error().decode())
+    return rc
+
+
+def glibc_seeds(core_seed):
+    out = (C.c_int32 * 4)()
+    _check(lib().pt_glibc_seeds(core_seed, out))
+    return list(out)
+
+
+def glibc_initial_infected(core_seed, n_people, n):
+    out = np.zeros(max(n, 1), np.int32)
+    _check(lib().pt_glibc_initial_infected(core_seed, n_people, n, out.ctypes.data))
+    return out[:n]
+
+
+class PandemicSim:
+    """One simulation on one GPU (the reference's PandemicSim object)."""
+
+    def __init__(self, params, core_seed, device=0):
+        self.params = params
+        self.core_seed = core_seed
+        self.seeds = glibc_seeds(core_seed)
+        key = (C.c_int32 * 2)(self.seeds[0], self.seeds[1])
+        h = C.c_void_p()
+        _check(lib().pt_sim_create(C.byref(params), key, device, C.byref(h)))
+        self.h = h
+        self.n_people = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pt_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # setupSim (PandemicSim.cu:146-208)
+    def setup_sim(self):
+        n = C.c_int64(0)
+        _check(lib().pt_sim_setup(self.h, C.byref(n)))
+        self.n_people = int(n.value)
+        p = self.params
+        ids = glibc_initial_infected(self.core_seed, self.n_people, p.init_pandemic + p.init_seasonal)
+        _check(lib().pt_sim_seed_infections(self.h, ids.ctypes.data, p.init_pandemic, p.init_seasonal))
+        return self.n_people
+
+    def step(self):
+        counts = np.zeros(16, np.int32)
+        inf = C.c_int32(0)
+        rc = _check(lib().pt_sim_step(self.h, counts.ctypes.data, C.byref(inf)))
+        return rc, counts, int(inf.value)
+
+    # runToCompletion loop (PandemicSim.cu:877-967)
+    def run(self):
+        md = max(self.params.max_days, 1)
+        counts = np.zeros((md, 16), np.int32)
+        inf = np.zeros(md, np.int32)
+        days = C.c_int32(0)
+        _check(lib().pt_sim_run(self.h, counts.ctypes.data, inf.ctypes.data, C.byref(days)))
+        d = int(days.value)
+        return counts[:d].copy(), inf[:d].copy()
+
+    def generation_offsets(self):
+        md = self.params.max_days
+        cp = np.zeros(md + 1, np.int32)
+        cs = np.zeros(md + 1, np.int32)
+        _check(lib().pt_sim_generation_offsets(self.h, cp.ctypes.data, cs.ctypes.data, md + 1))
+        return cp, cs
+
+    def array(self, which):
+        nbytes = int(lib().pt_sim_array_bytes(self.h, which))
+        if nbytes < 0:
+            raise PTError(lib().pt_last_error().decode())
+        dt = np.uint8 if which in _U8_ARRAYS else np.int32
+        out = np.zeros(nbytes // np.dtype(dt).itemsize, dt)
+        if nbytes:
+            _check(lib().pt_sim_copy_array(self.h, which, out.ctypes.data, nbytes))
+        return out
+
+    def scalars(self):
+        out = np.zeros(8, np.int64)
+        _check(lib().pt_sim_get_scalars(self.h, out.ctypes.data))
+        return out
+
+    def snapshot(self):
+        _check(lib().pt_sim_snapshot(self.h))
+
+    def restore(self):
+        _check(lib().pt_sim_restore(self.h))
+
+    def enable_timing(self, on=True):
+        _check(lib().pt_sim_enable_timing(self.h, 1 if on else 0))
+
+    def kernel_times(self):
+        n = 32
+        ms = np.zeros(n, np.float64)
+        cnt = np.zeros(n, np.int64)
+        k = _check(lib().pt_sim_kernel_times(self.h, ms.ctypes.data, cnt.ctypes.data, n))
+        return {lib().pt_sim_kernel_name(i).decode(): (float(ms[i]), int(cnt[i])) for i in range(k)}
+
+    def synchronize(self):
+        _check(lib().pt_sim_synchronize(self.h))
+
+    def last_run_ms(self):
+        return float(lib().pt_sim_last_run_ms(self.h))

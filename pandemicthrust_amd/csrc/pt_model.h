@@ -1,0 +1,253 @@
+// pt_model.h — model tables and per-agent device functions of the daily loop.
+//
+// The functions here are the per-thread bodies of the reference's device
+// functions (citations relative to PandemicThrust/PandemicThrust/), written
+// for gfx950.  Floating point follows the reference's single-precision
+// operation order; the library is built with -ffp-contract=off and IEEE
+// division so that results are bit-identical to the reference semantics.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pt {
+
+constexpr int kTypes = 14;
+constexpr int kHhRows = 9;
+constexpr int kChildRows = 5;
+constexpr int kAgeAdult = 5;
+constexpr int kCulmination = 10;
+constexpr int kFirstErrandRow = 9;
+constexpr int kTypeAfterschool = 8;
+constexpr int kTypePreschool = 3;
+constexpr int kWdHours = 2, kWeHours = 10;
+constexpr int kWdErrands = 2, kWeErrands = 3;
+constexpr int kContactsWd = 8, kContactsWe = 5;
+enum ContactType : int { CT_NONE = 0, CT_WORKPLACE, CT_SCHOOL, CT_ERRAND, CT_AFTERSCHOOL, CT_HOME };
+constexpr uint8_t kRecovered = 0, kSusceptible = 1, kInfected = 2;
+constexpr uint8_t kDayNone = 255, kGenNone = 255;
+
+// Model tables resident in device memory, one copy per simulation handle.
+struct Model {
+    float hh_cdf[kHhRows];
+    int32_t hh_adults[kHhRows];
+    int32_t hh_children[kHhRows];
+    float child_cdf[kChildRows];
+    float wp_pdf[kTypes];
+    int32_t wp_count[kTypes];
+    int32_t wp_offset[kTypes];
+    int32_t wp_max[kTypes];
+    float wd_pdf[kTypes];
+    float we_pdf[kTypes];
+    int32_t wd_assign[4][2];
+    int32_t we_assign[6][2];
+    float shed[6][10];
+    float factor[2];
+    float pct;
+    int32_t pad0;
+    uint64_t key0, key1;  // Threefry2x64 key {SEED[0], SEED[1]}
+    int32_t N, H, W;
+    // Dense errand-table layout: every errand destination lies in
+    // [base, W), so bucket (hour, loc) maps to hour * E + (loc - base).
+    int32_t wd_base, wd_E;
+    int32_t we_base, we_E;
+};
+
+// ---------------- Threefry (Salmon et al., SC'11) ----------------
+__host__ __device__ __forceinline__ uint64_t rotl64(uint64_t x, unsigned n) {
+    return (x << n) | (x >> (64 - n));
+}
+__host__ __device__ __forceinline__ uint32_t rotl32(uint32_t x, unsigned n) {
+    return (x << n) | (x >> (32 - n));
+}
+
+// Threefry2x64, 20 rounds, counter {c, c}: the reference always passes the
+// same 64-bit offset in both counter words (e.g. PandemicSim.cu:2340).
+struct U4 { uint32_t w[4]; };
+__device__ __forceinline__ U4 draw4(uint64_t c, uint64_t k0, uint64_t k1) {
+    const uint64_t k2 = 0x1BD11BDAA9FC1A22ull ^ k0 ^ k1;
+    uint64_t x0 = c + k0, x1 = c + k1;
+#define PT_R(n) x0 += x1; x1 = rotl64(x1, n); x1 ^= x0;
+#define PT_INJ(a, b, s) x0 += a; x1 += b + (uint64_t)(s);
+    PT_R(16) PT_R(42) PT_R(12) PT_R(31) PT_INJ(k1, k2, 1)
+    PT_R(16) PT_R(32) PT_R(24) PT_R(21) PT_INJ(k2, k0, 2)
+    PT_R(16) PT_R(42) PT_R(12) PT_R(31) PT_INJ(k0, k1, 3)
+    PT_R(16) PT_R(32) PT_R(24) PT_R(21) PT_INJ(k1, k2, 4)
+    PT_R(16) PT_R(42) PT_R(12) PT_R(31) PT_INJ(k2, k0, 5)
+#undef PT_R
+#undef PT_INJ
+    U4 r;
+    // Union layout of the reference (u32 word 2i = low half of output i).
+    r.w[0] = (uint32_t)x0;
+    r.w[1] = (uint32_t)(x0 >> 32);
+    r.w[2] = (uint32_t)x1;
+    r.w[3] = (uint32_t)(x1 >> 32);
+    return r;
+}
+
+// Threefry2x32, 20 rounds (the weekend errand-contact draw,
+// PandemicSim.cu:1895-1901).
+__device__ __forceinline__ void draw2x32(uint32_t c, uint32_t k0, uint32_t k1, uint32_t out[2]) {
+    const uint32_t k2 = 0x1BD11BDAu ^ k0 ^ k1;
+    uint32_t x0 = c + k0, x1 = c + k1;
+#define PT_R(n) x0 += x1; x1 = rotl32(x1, n); x1 ^= x0;
+#define PT_INJ(a, b, s) x0 += a; x1 += b + (uint32_t)(s);
+    PT_R(13) PT_R(15) PT_R(26) PT_R(6) PT_INJ(k1, k2, 1)
+    PT_R(17) PT_R(29) PT_R(16) PT_R(24) PT_INJ(k2, k0, 2)
+    PT_R(13) PT_R(15) PT_R(26) PT_R(6) PT_INJ(k0, k1, 3)
+    PT_R(17) PT_R(29) PT_R(16) PT_R(24) PT_INJ(k1, k2, 4)
+    PT_R(13) PT_R(15) PT_R(26) PT_R(6) PT_INJ(k2, k0, 5)
+#undef PT_R
+#undef PT_INJ
+    out[0] = x0;
+    out[1] = x1;
+}
+
+// (float)u / UNSIGNED_MAX: the divisor converts to 2^32 in float, so this
+// is an exact scaling of the round-to-nearest conversion.
+__device__ __forceinline__ float unit_float(uint32_t u) { return (float)u / 4294967296.0f; }
+
+// ---------------- setup draws ----------------
+// device_setup_fishHouseholdType (PandemicSim.cu:2360-2371)
+__device__ __forceinline__ int fish_household(const Model& M, uint32_t r) {
+    const float y = unit_float(r);
+    int row = 0;
+    while (y > M.hh_cdf[row] && row < kHhRows - 1) row++;
+    return row;
+}
+
+// device_setup_fishWorkplace (PandemicSim.cu:2374-2399)
+__device__ __forceinline__ int32_t fish_workplace(const Model& M, uint32_t r) {
+    float y = unit_float(r);
+    int row = 0;
+    while (M.wp_pdf[row] < y && row < kTypes - 1) {
+        y -= M.wp_pdf[row];
+        row++;
+    }
+    const float frac = y / M.wp_pdf[row];
+    const int type_count = M.wp_count[row];
+    int bn = (int)(frac * (float)type_count);
+    if (bn >= type_count) bn = type_count - 1;
+    return bn + M.wp_offset[row];
+}
+
+// device_setup_fishSchoolAndAge (PandemicSim.cu:2402-2442)
+__device__ __forceinline__ int32_t fish_school_and_age(const Model& M, uint32_t r, uint8_t* age) {
+    const float y = unit_float(r);
+    int row = 0;
+    while (row < kChildRows - 1 && y > M.child_cdf[row]) row++;
+    const int wp_type = row + kTypePreschool;
+    float frac;
+    if (row == 0) {
+        frac = y / M.child_cdf[0];
+    } else {
+        const float pdf_here = M.child_cdf[row] - M.child_cdf[row - 1];
+        const float y_here = y - M.child_cdf[row - 1];
+        frac = y_here / pdf_here;
+    }
+    const int type_count = M.wp_count[wp_type];
+    int bn = (int)(frac * (float)type_count);
+    if (bn >= type_count) bn = type_count - 1;
+    *age = (uint8_t)row;
+    return bn + M.wp_offset[wp_type];
+}
+
+// device_setup_assignWorkplaceOrSchool (PandemicSim.cu:2445-2457)
+__device__ __forceinline__ int32_t assign_workplace(const Model& M, uint32_t r, uint8_t* age) {
+    if (*age == kAgeAdult) return fish_workplace(M, r);
+    return fish_school_and_age(M, r, age);
+}
+
+// ---------------- errand draws ----------------
+// device_fishAfterschoolOrErrandDestination_weekday (PandemicSim.cu:2909-2947)
+__device__ __forceinline__ int32_t fish_weekday_dest(const Model& M, uint32_t r, bool adult) {
+    int bt = kTypeAfterschool;
+    float frac = unit_float(r);
+    if (adult) {
+        bt = kFirstErrandRow;
+        float row_pdf = M.wd_pdf[bt];
+        while (frac > row_pdf && bt < kTypes - 1) {
+            frac -= row_pdf;
+            bt++;
+            row_pdf = M.wd_pdf[bt];
+        }
+        frac = frac / row_pdf;
+    }
+    const int type_count = M.wp_count[bt];
+    uint32_t bn = (uint32_t)(frac * (float)type_count);
+    if (bn >= (uint32_t)type_count) bn = (uint32_t)(type_count - 1);
+    bn += (uint32_t)M.wp_offset[bt];
+    return (int32_t)bn;
+}
+
+// device_fishWeekendErrandDestination (PandemicSim.cu:1017-1044)
+__device__ __forceinline__ int32_t fish_weekend_dest(const Model& M, uint32_t r) {
+    float y = unit_float(r);
+    int row = kFirstErrandRow;
+    while (y > M.we_pdf[row] && row < kTypes - 1) {
+        y -= M.we_pdf[row];
+        row++;
+    }
+    const float frac = y / M.we_pdf[row];
+    const int type_count = M.wp_count[row];
+    int bn = (int)(frac * (float)type_count);
+    if (bn >= type_count) bn = type_count - 1;
+    return bn + M.wp_offset[row];
+}
+
+// Weekday errand destinations of one person from its two draws
+// (device_assignAfterschoolOrErrandDests_weekday, PandemicSim.cu:2950-2969),
+// returned as locations (hour 1's W offset is not added).
+__device__ __forceinline__ void weekday_dests(const Model& M, uint32_t r1, uint32_t r2, bool adult, int32_t& d1,
+                                              int32_t& d2) {
+    d1 = fish_weekday_dest(M, r1, adult);
+    d2 = adult ? fish_weekday_dest(M, r2, adult) : d1;
+}
+
+// device_generateWeekendErrands (PandemicSim.cu:980-1014): three distinct
+// hours and their destinations.
+__device__ __forceinline__ void weekend_errands(const Model& M, uint64_t off, int32_t hour[3], int32_t loc[3]) {
+    const U4 a = draw4(off, M.key0, M.key1);
+    const U4 b = draw4(off + 1, M.key0, M.key1);
+    const int h1 = (int)(a.w[0] % kWeHours);
+    int h2 = (int)(a.w[1] % kWeHours);
+    if (h2 == h1) h2 = (h2 + 1) % kWeHours;
+    int h3 = (int)(a.w[2] % kWeHours);
+    while (h3 == h1 || h3 == h2) h3 = (h3 + 1) % kWeHours;
+    hour[0] = h1;
+    hour[1] = h2;
+    hour[2] = h3;
+    loc[0] = fish_weekend_dest(M, a.w[3]);
+    loc[1] = fish_weekend_dest(M, b.w[0]);
+    loc[2] = fish_weekend_dest(M, b.w[1]);
+}
+
+// device_getWorkplaceMaxContacts (PandemicSim.cu:3492-3511)
+__device__ __forceinline__ int workplace_max_contacts(const Model& M, int32_t loc) {
+    const int id = loc % M.W;
+    int t = 0, off, cnt, mx;
+    do {
+        off = M.wp_offset[t];
+        cnt = M.wp_count[t];
+        mx = M.wp_max[t];
+        t++;
+    } while (id >= off + cnt && t < kTypes);
+    return mx;
+}
+
+// device_calculateInfectionProbability (PandemicSim.cu:2307-2326)
+__device__ __forceinline__ float infection_probability(const Model& M, int profile, int dd, int strain, int kval_sum) {
+    if (kval_sum == 0) kval_sum = 1;
+    const float alpha = M.shed[profile][dd];
+    const float avg = alpha * M.factor[strain];
+    return avg / (float)kval_sum;
+}
+
+// device_getInfectionProfile (PandemicSim.cu:3974-3988)
+__device__ __forceinline__ uint32_t infection_profile(const Model& M, uint32_t r) {
+    uint32_t prof = r % 3u + kInfected;
+    if (unit_float(r) > M.pct) prof += 3;
+    return prof;
+}
+
+}  // namespace pt

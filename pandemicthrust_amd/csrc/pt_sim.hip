@@ -1,0 +1,1625 @@
+// pt_sim.hip — gfx950 kernels of the PandemicThrust daily loop and the C ABI
+// of include/pandemic/pt_abi.h.
+//
+// Citations are relative to PandemicThrust/PandemicThrust/ of the reference.
+//
+// Device data (one copy per handle, structure of arrays in HBM):
+//   per agent   status/day/generation per strain and age (uint8),
+//               household id (int32); claim words per strain (uint64)
+//   tables      household offsets, workplace offsets + people, the day's
+//               errand offsets (dense bucket layout) + people
+//   day state   the reference's host counters (current day, rand_offset,
+//               saved errand offset, infected count) kept on the device so
+//               that a whole run is enqueued without host round trips.
+//
+// Infection resolution.  The reference applies infections with a racy
+// atomicCAS (PandemicSim.cu:3048-3067).  Here every successful attempt on a
+// victim that is susceptible at the start of the day posts
+// ((pos*8 + slot) << 32 | gen << 8 | profile) with a 64-bit atomicMin into
+// the victim's claim word for that strain; the smallest (infected-list
+// position, contact slot) wins, which is the outcome of processing infectors
+// in ascending position and contacts in slot order.  The first poster of a
+// claim word appends the victim to the strain's claim list; k_apply then
+// writes status/day/generation from the winning payload and clears the word.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pandemic/pt_abi.h"
+#include "pt_model.h"
+#include "pt_sort.h"
+
+using namespace pt;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return fail(PT_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kSweepThreads = 512;
+constexpr int kSweepPer = 16;                             // agents per thread (one uint4 per array)
+constexpr int kSweepTile = kSweepThreads * kSweepPer;     // 8192 agents per block
+constexpr int kPlanThreads = 1024;
+constexpr int kContactThreads = 256;
+
+// The reference's host-side loop counters (PandemicSim.h:73-76,
+// simRandOffsetsStruct_t) kept in device memory.
+struct DayState {
+    int32_t day;           // current_day
+    int32_t done;          // the loop has ended
+    int32_t days_run;      // rows of output_infected_stats.csv produced
+    int32_t n_infected;    // infected_count of the current day
+    uint64_t rand_offset;  // rand_offset
+    uint64_t errand_off;   // errand_randOffset of the current day
+    uint64_t contact_base; // rand_offset at the contact kernel
+    uint32_t n_claimed[2]; // claim-list lengths (pandemic, seasonal)
+    int32_t max_days;
+    int32_t fault;         // nonzero if a kernel saw data outside the model's range
+};
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// All 64 lanes must call: reserves n_mine slots per lane in a list whose
+// length lives at *counter; returns the lane's first slot.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, uint32_t n_mine, int lane) {
+    const uint32_t inc = wave_incl_scan(n_mine, lane);
+    const uint32_t total = __shfl(inc, 63, 64);
+    uint32_t base = 0;
+    if (lane == 0 && total) base = atomicAdd(counter, total);
+    base = __shfl(base, 0, 64);
+    return base + inc - n_mine;
+}
+
+// ============================ setup kernels ============================
+
+// K1/K2: household types, four per Threefry counter g = 0..H/4
+// (hh_peopleCount_functor PandemicSim.cu:3579-3605; kernel_householdTypeAssignment :2329-2357).
+__global__ __launch_bounds__(256) void k_household_types(const Model* __restrict__ Mp, uint8_t* __restrict__ types,
+                                                         unsigned long long* __restrict__ people) {
+    const Model& M = *Mp;
+    const int H = M.H;
+    __shared__ uint32_t s_w[4];
+    uint32_t cnt = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g <= H / 4; g += stride) {
+        const U4 u = draw4((uint64_t)g, M.key0, M.key1);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t h = 4 * g + i;
+            if (h < H) {
+                const int t = fish_household(M, u.w[i]);
+                types[h] = (uint8_t)t;
+                cnt += (uint32_t)(M.hh_adults[t] + M.hh_children[t]);
+            }
+        }
+    }
+    cnt = wave_sum(cnt);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) s_w[wave] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < (int)(blockDim.x / 64); ++w) t += s_w[w];
+        atomicAdd(people, t);
+    }
+}
+
+__global__ void k_household_sizes(const Model* __restrict__ Mp, const uint8_t* __restrict__ types,
+                                  uint32_t* __restrict__ sizes) {
+    const Model& M = *Mp;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h <= M.H; h += stride) {
+        if (h < M.H) {
+            const int t = types[h];
+            sizes[h] = (uint32_t)(M.hh_adults[t] + M.hh_children[t]);
+        } else {
+            sizes[h] = 0;
+        }
+    }
+}
+
+// K4: kernel_generateHouseholds (PandemicSim.cu:2490-2531): adults first.
+__global__ void k_generate_households(const Model* __restrict__ Mp, const uint8_t* __restrict__ types,
+                                      const int32_t* __restrict__ hh_off, int32_t* __restrict__ household,
+                                      uint8_t* __restrict__ age) {
+    const Model& M = *Mp;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h < M.H; h += stride) {
+        const int t = types[h];
+        const int off = hh_off[h];
+        const int na = M.hh_adults[t], nc = M.hh_children[t];
+        for (int a = 0; a < na; ++a) {
+            household[off + a] = (int32_t)h;
+            age[off + a] = kAgeAdult;
+        }
+        for (int c = 0; c < nc; ++c) {
+            household[off + na + c] = (int32_t)h;
+            age[off + na + c] = 0;
+        }
+    }
+}
+
+// K5: assignWorkplaceFunctor (PandemicSim.cu:3750-3778), four people per
+// counter workplace_off + g, g = 0..N/4.
+__global__ void k_assign_workplaces(const Model* __restrict__ Mp, uint64_t workplace_off, uint8_t* __restrict__ age,
+                                    uint32_t* __restrict__ wp_keys) {
+    const Model& M = *Mp;
+    const int N = M.N;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g <= N / 4; g += stride) {
+        const U4 u = draw4(workplace_off + (uint64_t)g, M.key0, M.key1);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t p = 4 * g + i;
+            if (p < N) {
+                uint8_t a = age[p];
+                const int32_t wp = assign_workplace(M, u.w[i], &a);
+                age[p] = a;
+                wp_keys[p] = (uint32_t)wp;
+            }
+        }
+    }
+}
+
+// K7: setup_initializeStatusArrays (PandemicSim.cu:3654-3664).
+__global__ void k_init_status(int64_t N, uint8_t* st_p, uint8_t* st_s, uint8_t* dy_p, uint8_t* dy_s, uint8_t* gn_p,
+                              uint8_t* gn_s) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+        st_p[i] = kSusceptible;
+        st_s[i] = kSusceptible;
+        dy_p[i] = kDayNone;
+        dy_s[i] = kDayNone;
+        gn_p[i] = kGenNone;
+        gn_s[i] = kGenNone;
+    }
+}
+
+__global__ void k_fill_u64(uint64_t* p, int64_t n, uint64_t v) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// setup_initialInfected (PandemicSim.cu:708-758).
+__global__ void k_seed_infections(const int32_t* __restrict__ ids, int n_p, int n_s, uint8_t* st_p, uint8_t* st_s,
+                                  uint8_t* dy_p, uint8_t* dy_s, uint8_t* gn_p, uint8_t* gn_s) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_p + n_s) return;
+    const int id = ids[i];
+    if (i < n_p) {
+        st_p[id] = kInfected;
+        dy_p[id] = 0;
+        gn_p[id] = 0;
+    } else {
+        st_s[id] = kInfected;
+        dy_s[id] = 0;
+        gn_s[id] = 0;
+    }
+}
+
+// ============================ daily kernels ============================
+
+// D1 + D2 fused: recover (day <= current_day - 10), count the 8 status codes
+// per strain and stage the ids of infected agents per block.
+// (kernel_countInfectedStatusAndRecover PandemicSim.cu:2097-2192,
+//  daily_buildInfectedArray_global :2207-2224.)
+__global__ __launch_bounds__(kSweepThreads) void k_sweep(int64_t N, DayState* __restrict__ st,
+                                                         uint8_t* __restrict__ st_p, uint8_t* __restrict__ st_s,
+                                                         const uint8_t* __restrict__ dy_p,
+                                                         const uint8_t* __restrict__ dy_s,
+                                                         uint32_t* __restrict__ blk_counts,
+                                                         uint32_t* __restrict__ blk_ninf,
+                                                         int32_t* __restrict__ staging) {
+    if (st->done) return;
+    const int dtr = st->day - kCulmination;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t a0 = (int64_t)blockIdx.x * kSweepTile + (int64_t)tid * kSweepPer;
+    uint8_t sp[kSweepPer], ss[kSweepPer];
+    const bool full = a0 + kSweepPer <= N;
+    if (full) {
+        const uint4 vp = *reinterpret_cast<const uint4*>(st_p + a0);
+        const uint4 vs = *reinterpret_cast<const uint4*>(st_s + a0);
+        memcpy(sp, &vp, 16);
+        memcpy(ss, &vs, 16);
+    } else {
+#pragma unroll
+        for (int i = 0; i < kSweepPer; ++i) {
+            sp[i] = (a0 + i < N) ? st_p[a0 + i] : kSusceptible;
+            ss[i] = (a0 + i < N) ? st_s[a0 + i] : kSusceptible;
+        }
+    }
+    if (dtr >= 0) {
+        bool anyp = false, anys = false;
+#pragma unroll
+        for (int i = 0; i < kSweepPer; ++i) {
+            anyp |= sp[i] >= kInfected;
+            anys |= ss[i] >= kInfected;
+        }
+        if (anyp) {
+            uint8_t dp[kSweepPer];
+            if (full) {
+                const uint4 v = *reinterpret_cast<const uint4*>(dy_p + a0);
+                memcpy(dp, &v, 16);
+            } else {
+#pragma unroll
+                for (int i = 0; i < kSweepPer; ++i) dp[i] = (a0 + i < N) ? dy_p[a0 + i] : kDayNone;
+            }
+            bool dirty = false;
+#pragma unroll
+            for (int i = 0; i < kSweepPer; ++i)
+                if (sp[i] >= kInfected && (int)dp[i] <= dtr) {
+                    sp[i] = kRecovered;
+                    dirty = true;
+                }
+            if (dirty) {
+                if (full) {
+                    uint4 v;
+                    memcpy(&v, sp, 16);
+                    *reinterpret_cast<uint4*>(st_p + a0) = v;
+                } else {
+                    for (int i = 0; i < kSweepPer; ++i)
+                        if (a0 + i < N) st_p[a0 + i] = sp[i];
+                }
+            }
+        }
+        if (anys) {
+            uint8_t ds[kSweepPer];
+            if (full) {
+                const uint4 v = *reinterpret_cast<const uint4*>(dy_s + a0);
+                memcpy(ds, &v, 16);
+            } else {
+#pragma unroll
+                for (int i = 0; i < kSweepPer; ++i) ds[i] = (a0 + i < N) ? dy_s[a0 + i] : kDayNone;
+            }
+            bool dirty = false;
+#pragma unroll
+            for (int i = 0; i < kSweepPer; ++i)
+                if (ss[i] >= kInfected && (int)ds[i] <= dtr) {
+                    ss[i] = kRecovered;
+                    dirty = true;
+                }
+            if (dirty) {
+                if (full) {
+                    uint4 v;
+                    memcpy(&v, ss, 16);
+                    *reinterpret_cast<uint4*>(st_s + a0) = v;
+                } else {
+                    for (int i = 0; i < kSweepPer; ++i)
+                        if (a0 + i < N) st_s[a0 + i] = ss[i];
+                }
+            }
+        }
+    }
+    // Counts: 8 bins per strain packed as bytes (at most 16 per thread).
+    uint64_t cp = 0, cs = 0;
+    uint32_t inf_mask = 0;
+#pragma unroll
+    for (int i = 0; i < kSweepPer; ++i) {
+        if (a0 + i < N) {
+            cp += 1ull << (8 * (sp[i] & 7));
+            cs += 1ull << (8 * (ss[i] & 7));
+            if (sp[i] >= kInfected || ss[i] >= kInfected) inf_mask |= 1u << i;
+        }
+    }
+    __shared__ uint32_t s_cnt[kSweepThreads / 64][16];
+    __shared__ uint32_t s_inf[kSweepThreads / 64];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const uint32_t vp = wave_sum((uint32_t)((cp >> (8 * b)) & 0xFF));
+        const uint32_t vs = wave_sum((uint32_t)((cs >> (8 * b)) & 0xFF));
+        if (lane == 0) {
+            s_cnt[wave][b] = vp;
+            s_cnt[wave][8 + b] = vs;
+        }
+    }
+    const uint32_t ninf = (uint32_t)__popc(inf_mask);
+    const uint32_t inc = wave_incl_scan(ninf, lane);
+    if (lane == 63) s_inf[wave] = inc;
+    __syncthreads();
+    uint32_t wbase = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kSweepThreads / 64; ++w) {
+        if (w < wave) wbase += s_inf[w];
+        total += s_inf[w];
+    }
+    uint32_t pos = wbase + inc - ninf;
+    int32_t* out = staging + (int64_t)blockIdx.x * kSweepTile;
+#pragma unroll
+    for (int i = 0; i < kSweepPer; ++i)
+        if (inf_mask & (1u << i)) out[pos++] = (int32_t)(a0 + i);
+    if (tid < 16) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int w = 0; w < kSweepThreads / 64; ++w) s += s_cnt[w][tid];
+        blk_counts[(int64_t)blockIdx.x * 16 + tid] = s;
+    }
+    if (tid == 0) blk_ninf[blockIdx.x] = total;
+}
+
+// Day plan (one block): daily counts, the infected count and block offsets
+// of the compaction, the stop rule (PandemicSim.cu:899-901) and the RNG
+// offsets of the day (:3030-3040 weekday, :1409-1415 weekend, :1278-1279 and
+// :1393-1394 contacts).
+__global__ __launch_bounds__(kPlanThreads) void k_plan(DayState* __restrict__ st, int nblk,
+                                                       const uint32_t* __restrict__ blk_counts,
+                                                       const uint32_t* __restrict__ blk_ninf,
+                                                       uint32_t* __restrict__ blk_off, int32_t* __restrict__ out_counts,
+                                                       int32_t* __restrict__ out_inf, int64_t N) {
+    if (st->done) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int NW = kPlanThreads / 64;
+    __shared__ uint32_t s_red[NW][16];
+    __shared__ uint32_t s_w[NW];
+    __shared__ uint32_t s_carry;
+    uint32_t c[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) c[i] = 0;
+    for (int b = tid; b < nblk; b += kPlanThreads)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) c[i] += blk_counts[(int64_t)b * 16 + i];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const uint32_t v = wave_sum(c[i]);
+        if (lane == 0) s_red[wave][i] = v;
+    }
+    if (tid == 0) s_carry = 0;
+    __syncthreads();
+    const int day = st->day;
+    if (tid < 16) {
+        uint32_t s = 0;
+        for (int w = 0; w < NW; ++w) s += s_red[w][tid];
+        out_counts[day * 16 + tid] = (int32_t)s;
+    }
+    for (int b0 = 0; b0 < nblk; b0 += kPlanThreads) {
+        const int b = b0 + tid;
+        const uint32_t v = b < nblk ? blk_ninf[b] : 0u;
+        const uint32_t inc = wave_incl_scan(v, lane);
+        if (lane == 63) s_w[wave] = inc;
+        __syncthreads();
+        uint32_t wbase = 0, tot = 0;
+        for (int w = 0; w < NW; ++w) {
+            if (w < wave) wbase += s_w[w];
+            tot += s_w[w];
+        }
+        const uint32_t carry = s_carry;
+        if (b < nblk) blk_off[b] = carry + wbase + inc - v;
+        __syncthreads();
+        if (tid == 0) s_carry = carry + tot;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const int32_t I = (int32_t)s_carry;
+        st->n_infected = I;
+        st->days_run = day + 1;
+        out_inf[day] = I;
+        if (I == 0) {
+            st->done = 1;
+        } else {
+            const bool weekend = (day % 7) >= 5;
+            st->errand_off = st->rand_offset;
+            st->rand_offset += weekend ? (uint64_t)(2 * N) : (uint64_t)(N / 2);
+            st->contact_base = st->rand_offset;
+            st->rand_offset += (uint64_t)6 * (uint64_t)I;
+        }
+    }
+}
+
+__global__ void k_compact(const DayState* __restrict__ st, const int32_t* __restrict__ staging,
+                          const uint32_t* __restrict__ blk_ninf, const uint32_t* __restrict__ blk_off,
+                          int32_t* __restrict__ infected) {
+    if (st->done) return;
+    const uint32_t n = blk_ninf[blockIdx.x];
+    const uint32_t off = blk_off[blockIdx.x];
+    const int32_t* src = staging + (int64_t)blockIdx.x * kSweepTile;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) infected[off + i] = src[i];
+}
+
+// D3w: kernel_assignWeekdayAfterschoolAndErrands (PandemicSim.cu:2972-3022).
+// Person pair (2g, 2g+1) shares counter errand_off + g.  Keys are dense:
+// hour * E + (loc - base); hour 0 block [0, N), hour 1 block [N, 2N).
+__global__ __launch_bounds__(256) void k_gen_weekday(const Model* __restrict__ Mp, DayState* __restrict__ st,
+                                                     const uint8_t* __restrict__ age, uint32_t* __restrict__ keys) {
+    if (st->done) return;
+    const Model& M = *Mp;
+    const int N = M.N;
+    const uint64_t off = st->errand_off;
+    const int base = M.wd_base, E = M.wd_E;
+    bool bad = false;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; 2 * g < N; g += stride) {
+        const U4 u = draw4(off + (uint64_t)g, M.key0, M.key1);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int64_t p = 2 * g + k;
+            if (p < N) {
+                const bool adult = age[p] == kAgeAdult;
+                int32_t d1, d2;
+                weekday_dests(M, u.w[2 * k], u.w[2 * k + 1], adult, d1, d2);
+                int32_t l1 = d1 - base, l2 = d2 - base;
+                if ((uint32_t)l1 >= (uint32_t)E || (uint32_t)l2 >= (uint32_t)E) {
+                    bad = true;
+                    l1 = min(max(l1, 0), E - 1);
+                    l2 = min(max(l2, 0), E - 1);
+                }
+                keys[p] = (uint32_t)l1;
+                keys[N + p] = (uint32_t)(E + l2);
+            }
+        }
+    }
+    if (bad) st->fault = 1;
+}
+
+// D3e: kernel_assignWeekendErrands (PandemicSim.cu:2082-2094); person p uses
+// counters errand_off + 2p, +1.  Keys: hour * E + (loc - base), 3 per person.
+__global__ __launch_bounds__(256) void k_gen_weekend(const Model* __restrict__ Mp, DayState* __restrict__ st,
+                                                     uint32_t* __restrict__ keys) {
+    if (st->done) return;
+    const Model& M = *Mp;
+    const int N = M.N;
+    const uint64_t off = st->errand_off;
+    const int base = M.we_base, E = M.we_E;
+    bool bad = false;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < N; p += stride) {
+        int32_t hour[3], loc[3];
+        weekend_errands(M, off + (uint64_t)(int64_t)(2 * (int32_t)p), hour, loc);
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            int32_t l = loc[e] - base;
+            if ((uint32_t)l >= (uint32_t)E) {
+                bad = true;
+                l = min(max(l, 0), E - 1);
+            }
+            keys[3 * p + e] = (uint32_t)(hour[e] * E + l);
+        }
+    }
+    if (bad) st->fault = 1;
+}
+
+// device_selectRandomPersonFromLocation (PandemicSim.cu:2028-2071) with
+// device_getVictimAtIndex (:2016-2025): household members are consecutive ids.
+__device__ __forceinline__ int select_person(int32_t infector, int32_t loc_off, int32_t loc_count, uint32_t r,
+                                             int type, const int32_t* __restrict__ people, int32_t& victim) {
+    if (loc_count > 1) {
+        int vo = (int)(r % (uint32_t)loc_count);
+        int32_t v = (type == CT_HOME) ? loc_off + vo : people[loc_off + vo];
+        if (v == infector) {
+            vo = (vo + 1) % loc_count;
+            v = (type == CT_HOME) ? loc_off + vo : people[loc_off + vo];
+        }
+        victim = v;
+        return type;
+    }
+    victim = loc_count;
+    return CT_NONE;
+}
+
+struct AgentArrays {
+    const uint8_t* age;
+    const int32_t* household;
+    const int32_t* hh_off;
+    const int32_t* wp_off;
+    const int32_t* wp_people;
+    const int32_t* er_off;     // dense
+    const int32_t* er_people;
+    const uint8_t* st_p;
+    const uint8_t* st_s;
+    const uint8_t* dy_p;
+    const uint8_t* dy_s;
+    const uint8_t* gn_p;
+    const uint8_t* gn_s;
+    uint64_t* claim_p;
+    uint64_t* claim_s;
+    int32_t* list_p;
+    int32_t* list_s;
+    int32_t* trace_victim;     // optional [I*C]
+    uint8_t* trace_type;       // optional [I*C]
+};
+
+// Strain state of an infector (device_processContacts, PandemicSim.cu:3125-3175):
+// profile and shedding day if infectious today, else -1.
+struct Infectivity {
+    int prof_p, dd_p, prof_s, dd_s;
+    uint8_t gen_p, gen_s;
+};
+
+__device__ __forceinline__ Infectivity infectivity(const AgentArrays& A, int32_t idx, int day) {
+    Infectivity f;
+    f.prof_p = f.prof_s = -1;
+    f.dd_p = f.dd_s = 0;
+    f.gen_p = f.gen_s = kGenNone;
+    const uint8_t sp = A.st_p[idx], ss = A.st_s[idx];
+    if (sp >= kInfected) {
+        const int dd = day - (int)A.dy_p[idx];
+        if (dd >= 0 && dd < kCulmination) {
+            f.prof_p = sp - kInfected;
+            f.dd_p = dd;
+            f.gen_p = (uint8_t)(A.gn_p[idx] + 1);
+        }
+    }
+    if (ss >= kInfected) {
+        const int dd = day - (int)A.dy_s[idx];
+        if (dd >= 0 && dd < kCulmination) {
+            f.prof_s = ss - kInfected;
+            f.dd_s = dd;
+            f.gen_s = (uint8_t)(A.gn_s[idx] + 1);
+        }
+    }
+    return f;
+}
+
+// Per-contact draws and claims (device_processContacts PandemicSim.cu:3177-3231
+// with the canonical resolution described at the top of this file).
+template <int C>
+__device__ __forceinline__ void process_contacts(const Model& M, const AgentArrays& A, const Infectivity& f, int kval_sum,
+                                                 const int32_t (&vic)[8], const int (&typ)[8], uint64_t M2,
+                                                 uint32_t pos, bool (&newp)[8], bool (&news)[8]) {
+    const float prob_p = f.prof_p >= 0 ? infection_probability(M, f.prof_p, f.dd_p, 0, kval_sum) : -1.f;
+    const float prob_s = f.prof_s >= 0 ? infection_probability(M, f.prof_s, f.dd_s, 1, kval_sum) : -1.f;
+    uint32_t u[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const U4 w = draw4(M2 + (uint64_t)q, M.key0, M.key1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) u[4 * q + j] = w.w[j];
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const float kv = typ[c] == CT_NONE ? 0.f : 1.f;
+        const int32_t v = vic[c];
+        if (unit_float(u[2 * c]) < prob_p * kv) {
+            if (A.st_p[v] == kSusceptible) {
+                const uint32_t prof = infection_profile(M, u[2 * c] ^ u[(2 * c + 1) % 16]);
+                const uint64_t claim = ((uint64_t)(pos * 8u + (uint32_t)c) << 32) | ((uint32_t)f.gen_p << 8) | prof;
+                const uint64_t old = atomicMin((unsigned long long*)&A.claim_p[v], (unsigned long long)claim);
+                newp[c] = old == ~0ull;
+            }
+        }
+        if (unit_float(u[2 * c + 1]) < prob_s * kv) {
+            if (A.st_s[v] == kSusceptible) {
+                const uint32_t prof = infection_profile(M, u[2 * c + 1] ^ u[(2 * c + 2) % 16]);
+                const uint64_t claim = ((uint64_t)(pos * 8u + (uint32_t)c) << 32) | ((uint32_t)f.gen_s << 8) | prof;
+                const uint64_t old = atomicMin((unsigned long long*)&A.claim_s[v], (unsigned long long)claim);
+                news[c] = old == ~0ull;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void append_claims(DayState* st, const AgentArrays& A, const int32_t (&vic)[8],
+                                              const bool (&newp)[8], const bool (&news)[8], int lane) {
+    uint32_t np = 0, ns = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        np += newp[c];
+        ns += news[c];
+    }
+    uint32_t ip = wave_append(&st->n_claimed[0], np, lane);
+    uint32_t is = wave_append(&st->n_claimed[1], ns, lane);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        if (newp[c]) A.list_p[ip++] = vic[c];
+        if (news[c]) A.list_s[is++] = vic[c];
+    }
+}
+
+// Whether the infector can infect at all today: with alpha*factor <= 0 every
+// draw y >= 0 fails y < p*kval, so contacts need not be generated.
+__device__ __forceinline__ bool can_infect(const Model& M, const Infectivity& f) {
+    const bool p = f.prof_p >= 0 && M.shed[f.prof_p][f.dd_p] * M.factor[0] > 0.f;
+    const bool s = f.prof_s >= 0 && M.shed[f.prof_s][f.dd_s] * M.factor[1] > 0.f;
+    return p || s;
+}
+
+// D6w + D7: kernel_doWeekday (PandemicSim.cu:3236-3308) with
+// device_makeContacts_weekday (:1669-1829).
+__global__ __launch_bounds__(kContactThreads) void k_contacts_weekday(const Model* __restrict__ Mp,
+                                                                      DayState* __restrict__ st,
+                                                                      const int32_t* __restrict__ infected,
+                                                                      AgentArrays A, uint64_t workplace_off) {
+    if (st->done) return;
+    const Model& M = *Mp;
+    const int I = st->n_infected;
+    const int day = st->day;
+    const uint64_t cbase = st->contact_base;
+    const uint64_t eoff = st->errand_off;
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < I; b0 += stride) {
+        const int64_t pos = b0 + threadIdx.x;
+        int32_t vic[8];
+        int typ[8];
+        bool newp[8], news[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            vic[c] = -1;
+            typ[c] = CT_NONE;
+            newp[c] = news[c] = false;
+        }
+        if (pos < I) {
+            const int32_t idx = infected[pos];
+            const Infectivity f = infectivity(A, idx, day);
+            if (can_infect(M, f) || A.trace_victim) {
+                const uint64_t Mc = cbase + (uint64_t)(6 * pos);
+                const U4 ua = draw4(Mc, M.key0, M.key1);
+                const U4 ub = draw4(Mc + 1, M.key0, M.key1);
+                const uint32_t u[8] = {ua.w[0], ua.w[1], ua.w[2], ua.w[3], ub.w[0], ub.w[1], ub.w[2], ub.w[3]};
+                const uint8_t myage = A.age[idx];
+                const bool adult = myage == kAgeAdult;
+                // household
+                const int32_t hh = A.household[idx];
+                const int32_t hoff = A.hh_off[hh];
+                const int32_t hcnt = A.hh_off[hh + 1] - hoff;
+                // workplace (device_recalcWorkplace PandemicSim.cu:3872-3890)
+                const U4 uw = draw4(workplace_off + (uint64_t)(idx / 4), M.key0, M.key1);
+                uint8_t a2 = myage;
+                const int32_t wp = assign_workplace(M, uw.w[idx % 4], &a2);
+                const int32_t woff = A.wp_off[wp];
+                const int32_t wcnt = A.wp_off[wp + 1] - woff;
+                const int wpd = workplace_max_contacts(M, wp);
+                // errands (device_recalc_weekdayErrandDests_assignProfile :3667-3705)
+                const U4 ue = draw4(eoff + (uint64_t)(idx / 2), M.key0, M.key1);
+                const int rs = 2 * (idx % 2);
+                int32_t d1, d2;
+                weekday_dests(M, ue.w[rs], ue.w[rs + 1], adult, d1, d2);
+                const int prof = adult ? (int)((ue.w[rs] ^ ue.w[rs + 1]) % 3u) : 3;
+                const int h0 = M.wd_assign[prof][0], h1 = M.wd_assign[prof][1];
+                const int32_t k0 = d1 - M.wd_base, k1 = M.wd_E + d2 - M.wd_base;
+                const int32_t e0off = A.er_off[k0], e0cnt = A.er_off[k0 + 1] - e0off;
+                const int32_t e1off = A.er_off[k1], e1cnt = A.er_off[k1 + 1] - e1off;
+                const int wtype = adult ? CT_WORKPLACE : CT_SCHOOL;
+                const int etype = adult ? CT_ERRAND : CT_AFTERSCHOOL;
+                int kval = 0;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    int t;
+                    if (c < 3) {
+                        t = select_person(idx, hoff, hcnt, u[c], CT_HOME, nullptr, vic[c]);
+                    } else if (c < 3 + wpd) {
+                        t = select_person(idx, woff, wcnt, u[c], wtype, A.wp_people, vic[c]);
+                    } else {
+                        const int e = c - 3 - wpd;
+                        if (e < h0)
+                            t = select_person(idx, e0off, e0cnt, u[c], etype, A.er_people, vic[c]);
+                        else if (e < h0 + h1)
+                            t = select_person(idx, e1off, e1cnt, u[c], etype, A.er_people, vic[c]);
+                        else {
+                            vic[c] = -1;
+                            t = CT_NONE;
+                        }
+                    }
+                    typ[c] = t;
+                    kval += t != CT_NONE;
+                }
+                if (A.trace_victim) {
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        A.trace_victim[pos * 8 + c] = vic[c];
+                        A.trace_type[pos * 8 + c] = (uint8_t)typ[c];
+                    }
+                }
+                if (can_infect(M, f))
+                    process_contacts<8>(M, A, f, kval, vic, typ, Mc + 2, (uint32_t)pos, newp, news);
+            }
+        }
+        append_claims(st, A, vic, newp, news, lane);
+    }
+}
+
+// D6e + D7: kernel_doWeekend (PandemicSim.cu:3311-3379) with
+// device_makeContacts_weekend (:1833-1950).
+__global__ __launch_bounds__(kContactThreads) void k_contacts_weekend(const Model* __restrict__ Mp,
+                                                                      DayState* __restrict__ st,
+                                                                      const int32_t* __restrict__ infected,
+                                                                      AgentArrays A) {
+    if (st->done) return;
+    const Model& M = *Mp;
+    const int I = st->n_infected;
+    const int day = st->day;
+    const uint64_t cbase = st->contact_base;
+    const uint64_t eoff = st->errand_off;
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < I; b0 += stride) {
+        const int64_t pos = b0 + threadIdx.x;
+        int32_t vic[8];
+        int typ[8];
+        bool newp[8], news[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            vic[c] = -1;
+            typ[c] = CT_NONE;
+            newp[c] = news[c] = false;
+        }
+        if (pos < I) {
+            const int32_t idx = infected[pos];
+            const Infectivity f = infectivity(A, idx, day);
+            if (can_infect(M, f) || A.trace_victim) {
+                const uint64_t Mc = cbase + (uint64_t)(6 * pos);
+                const U4 u = draw4(Mc, M.key0, M.key1);
+                const int32_t hh = A.household[idx];
+                const int32_t hoff = A.hh_off[hh];
+                const int32_t hcnt = A.hh_off[hh + 1] - hoff;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) typ[c] = select_person(idx, hoff, hcnt, u.w[c], CT_HOME, nullptr, vic[c]);
+                const int prof = (int)(u.w[3] % 6u);
+                uint32_t v32[2];
+                draw2x32((uint32_t)(Mc + 1), (uint32_t)M.key0, (uint32_t)M.key1, v32);
+                int32_t hour[3], loc[3];
+                weekend_errands(M, eoff + (uint64_t)(int64_t)(2 * idx), hour, loc);
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const int slot = M.we_assign[prof][k];
+                    const int hsel = slot == 0 ? hour[0] : (slot == 1 ? hour[1] : hour[2]);
+                    const int lsel = slot == 0 ? loc[0] : (slot == 1 ? loc[1] : loc[2]);
+                    const int32_t key = hsel * M.we_E + (lsel - M.we_base);
+                    const int32_t eoff2 = A.er_off[key], ecnt = A.er_off[key + 1] - eoff2;
+                    typ[3 + k] = select_person(idx, eoff2, ecnt, v32[k], CT_ERRAND, A.er_people, vic[3 + k]);
+                }
+                int kval = 0;
+#pragma unroll
+                for (int c = 0; c < 5; ++c) kval += typ[c] != CT_NONE;
+                if (A.trace_victim) {
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) {
+                        A.trace_victim[pos * 5 + c] = vic[c];
+                        A.trace_type[pos * 5 + c] = (uint8_t)typ[c];
+                    }
+                }
+                if (can_infect(M, f))
+                    process_contacts<5>(M, A, f, kval, vic, typ, Mc + 2, (uint32_t)pos, newp, news);
+            }
+        }
+        append_claims(st, A, vic, newp, news, lane);
+    }
+}
+
+// Winners write status = profile, day = current_day + 1, generation
+// (device_setInfectionStatus PandemicSim.cu:3048-3067); claim words reset.
+__global__ void k_apply(const DayState* __restrict__ st, uint64_t* __restrict__ claim_p,
+                        uint64_t* __restrict__ claim_s, const int32_t* __restrict__ list_p,
+                        const int32_t* __restrict__ list_s, uint8_t* __restrict__ st_p, uint8_t* __restrict__ st_s,
+                        uint8_t* __restrict__ dy_p, uint8_t* __restrict__ dy_s, uint8_t* __restrict__ gn_p,
+                        uint8_t* __restrict__ gn_s) {
+    if (st->done) return;
+    const uint32_t np = st->n_claimed[0], ns = st->n_claimed[1];
+    const uint8_t dset = (uint8_t)(st->day + 1);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (int64_t)np + ns; i += stride) {
+        if (i < np) {
+            const int32_t v = list_p[i];
+            const uint64_t c = claim_p[v];
+            st_p[v] = (uint8_t)(c & 0xFF);
+            dy_p[v] = dset;
+            gn_p[v] = (uint8_t)((c >> 8) & 0xFF);
+            claim_p[v] = ~0ull;
+        } else {
+            const int32_t v = list_s[i - np];
+            const uint64_t c = claim_s[v];
+            st_s[v] = (uint8_t)(c & 0xFF);
+            dy_s[v] = dset;
+            gn_s[v] = (uint8_t)((c >> 8) & 0xFF);
+            claim_s[v] = ~0ull;
+        }
+    }
+}
+
+__global__ void k_end_day(DayState* st) {
+    if (st->done) return;
+    st->day += 1;
+    st->n_claimed[0] = 0;
+    st->n_claimed[1] = 0;
+    if (st->day >= st->max_days) st->done = 1;
+}
+
+// Generation histograms (final_countReproduction, PandemicSim.cu:2236-2260).
+__global__ void k_gen_hist(const uint8_t* __restrict__ gn_p, const uint8_t* __restrict__ gn_s, int64_t N,
+                           unsigned long long* __restrict__ hist) {
+    __shared__ uint32_t s_h[512];
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) s_h[i] = 0;
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+        atomicAdd(&s_h[gn_p[i]], 1u);
+        atomicAdd(&s_h[256 + gn_s[i]], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 512; i += blockDim.x)
+        if (s_h[i]) atomicAdd(&hist[i], (unsigned long long)s_h[i]);
+}
+
+// ============================ host side ============================
+
+// The reference model tables (PandemicSim.cu:329-481; simParameters.h).
+void base_tables(Model& m) {
+    const float hh_cdf[9] = {0.279f, 0.319f, 0.628f, 0.671f, 0.8f, 0.812f, 0.939f, 0.944f, 1.0f};
+    const int hh_a[9] = {1, 1, 2, 1, 2, 1, 2, 1, 2};
+    const int hh_c[9] = {0, 1, 0, 2, 1, 3, 2, 4, 3};
+    const float child_cdf[5] = {0.24f, 0.47f, 0.72f, 0.85f, 1.0f};
+    const float wp_pdf[14] = {0.06586f, 0.05802f, 0.30227f, 0.0048f, 0.00997f, 0.203f, 0.09736f,
+                              0.10598f, 0.00681f, 0.02599f, 0.f,     0.08749f, 0.03181f, 0.00064f};
+    const int wp_max[14] = {3, 3, 3, 2, 2, 3, 3, 2, 2, 2, 0, 2, 2, 2};
+    const float shed[6][10] = {
+        {0.007339835f, 0.332600216f, 0.501192066f, 0.142183447f, 0.015675154f, 0.000967407f, 4.055E-05f,
+         1.29105E-06f, 3.34836E-08f, 7.41011E-10f},
+        {0.002533572f, 0.348252834f, 0.498210218f, 0.130145145f, 0.018421298f, 0.002158374f, 0.000245489f,
+         2.88922E-05f, 3.61113E-06f, 4.83901E-07f},
+        {0.05927385f, 0.314171688f, 0.411588802f, 0.187010054f, 0.026934715f, 0.001013098f, 7.78449E-06f,
+         9.29441E-09f, 1.29796E-12f, 0},
+        {0.04687299f, 0.248505983f, 0.30307952f, 0.211008627f, 0.11087006f, 0.049241932f, 0.019562658f,
+         0.007179076f, 0.002482875f, 0.000820094f},
+        {0.028667712f, 0.283445338f, 0.319240133f, 0.190123057f, 0.093989959f, 0.044155659f, 0.020682822f,
+         0.009841839f, 0.00479234f, 0.002393665f},
+        {0.087866042f, 0.223005225f, 0.258992749f, 0.208637267f, 0.127489076f, 0.061148649f, 0.023406737f,
+         0.007216643f, 0.001802145f, 0.00036581f}};
+    memcpy(m.hh_cdf, hh_cdf, sizeof hh_cdf);
+    memcpy(m.hh_adults, hh_a, sizeof hh_a);
+    memcpy(m.hh_children, hh_c, sizeof hh_c);
+    memcpy(m.child_cdf, child_cdf, sizeof child_cdf);
+    memcpy(m.wp_pdf, wp_pdf, sizeof wp_pdf);
+    memcpy(m.wp_max, wp_max, sizeof wp_max);
+    for (int t = 0; t < kTypes; ++t) m.wd_pdf[t] = m.we_pdf[t] = 0.f;
+    m.wd_pdf[9] = 0.61919f;
+    m.wd_pdf[11] = 0.27812f;
+    m.wd_pdf[12] = 0.06601f;
+    m.wd_pdf[13] = 0.03668f;
+    m.we_pdf[9] = 0.51493f;
+    m.we_pdf[11] = 0.25586f;
+    m.we_pdf[12] = 0.1162f;
+    m.we_pdf[13] = 0.113f;
+    const int wd_assign[4][2] = {{2, 0}, {0, 2}, {1, 1}, {wp_max[kTypeAfterschool], 0}};
+    const int we_assign[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+    memcpy(m.wd_assign, wd_assign, sizeof wd_assign);
+    memcpy(m.we_assign, we_assign, sizeof we_assign);
+    memcpy(m.shed, shed, sizeof shed);
+}
+
+const int kBaseCounts[kTypes] = {1000, 7000, 2400, 300, 100, 200, 100, 100, 300, 500, 0, 300, 400, 100};
+
+int round_half_up(double d) { return (int)floor(d + 0.5); }  // roundHalfUp_toInt, PandemicSim.cu:1662
+
+const char* kKernelNames[KC_COUNT] = {"sweep",         "plan",       "compact",   "gen_weekday",
+                                      "gen_weekend",   "sort_upsweep", "sort_scan", "sort_downsweep",
+                                      "offsets",       "contacts_weekday", "contacts_weekend", "apply",
+                                      "end_day",       "setup"};
+
+struct EventTimer : LaunchTimer {
+    hipStream_t stream = nullptr;
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<int, std::pair<size_t, size_t>>> pending;  // class, (begin event, end event)
+    size_t used = 0;
+    size_t open_begin = 0;
+    double ms[KC_COUNT] = {0};
+    int64_t launches[KC_COUNT] = {0};
+    hipEvent_t next() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            (void)hipEventCreate(&e);
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void begin(int) override {
+        if (!on) return;
+        open_begin = used;
+        (void)hipEventRecord(next(), stream);
+    }
+    void end(int cls) override {
+        if (!on) return;
+        const size_t e = used;
+        (void)hipEventRecord(next(), stream);
+        pending.push_back({cls, {open_begin, e}});
+    }
+    // Call after the stream is idle.
+    void collect() {
+        for (auto& p : pending) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, pool[p.second.first], pool[p.second.second]) == hipSuccess) {
+                ms[p.first] += t;
+                launches[p.first] += 1;
+            }
+        }
+        pending.clear();
+        used = 0;
+    }
+    void reset() {
+        pending.clear();
+        used = 0;
+        for (int i = 0; i < KC_COUNT; ++i) {
+            ms[i] = 0;
+            launches[i] = 0;
+        }
+    }
+    ~EventTimer() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+struct Scoped {
+    LaunchTimer* tm;
+    int cls;
+    Scoped(LaunchTimer* t, int c) : tm(t), cls(c) { tm->begin(cls); }
+    ~Scoped() { tm->end(cls); }
+};
+
+}  // namespace
+
+struct pt_sim {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    pt_params params{};
+    Model model{};
+    Model* d_model = nullptr;
+    int64_t N = 0;
+    int H = 0, W = 0;
+    bool setup_done = false, seeded = false, have_snapshot = false;
+    uint64_t workplace_off = 0, rand_after_setup = 0;
+    // agent arrays
+    uint8_t *st_p = nullptr, *st_s = nullptr, *dy_p = nullptr, *dy_s = nullptr, *gn_p = nullptr, *gn_s = nullptr;
+    uint8_t* age = nullptr;
+    uint8_t* snap = nullptr;  // 6N bytes
+    int32_t *household = nullptr, *hh_off = nullptr, *wp_off = nullptr, *wp_people = nullptr;
+    uint8_t* hh_types = nullptr;
+    int32_t *infected = nullptr, *staging = nullptr;
+    uint32_t* keys = nullptr;        // 3N
+    int32_t* er_people = nullptr;    // 3N
+    int32_t* er_off = nullptr;       // dense, max(2 wd_E, 10 we_E) + 1
+    uint64_t *claim_p = nullptr, *claim_s = nullptr;
+    int32_t *list_p = nullptr, *list_s = nullptr;
+    uint32_t *blk_counts = nullptr, *blk_ninf = nullptr, *blk_off = nullptr;
+    int nblk_sweep = 0;
+    DayState* d_state = nullptr;
+    int32_t *d_counts = nullptr, *d_inf = nullptr;  // [256*16], [256]
+    unsigned long long* d_scratch64 = nullptr;      // 512
+    TableScratch tables;
+    int host_day = 0;      // days enqueued since setup / restore
+    int last_weekend = -1; // layout of the errand table of the last day enqueued
+    int64_t last_table_n = 0;
+    EventTimer timer;
+    hipEvent_t run_begin = nullptr, run_end = nullptr;
+    double last_run_ms = 0;
+    int grid_contacts = 1024, grid_gen = 2048;
+};
+
+namespace {
+
+template <class T>
+int dev_alloc(T** p, int64_t count) {
+    if (count <= 0) count = 1;
+    if (hipMalloc((void**)p, (size_t)count * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        return fail(PT_ENOMEM, "hipMalloc of %lld bytes failed", (long long)(count * (int64_t)sizeof(T)));
+    }
+    return 0;
+}
+
+template <class T>
+void dev_free(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+// Buffers sized by the population (allocated in pt_sim_setup).
+void free_population(pt_sim* s) {
+    dev_free(s->st_p);
+    dev_free(s->st_s);
+    dev_free(s->dy_p);
+    dev_free(s->dy_s);
+    dev_free(s->gn_p);
+    dev_free(s->gn_s);
+    dev_free(s->age);
+    dev_free(s->snap);
+    dev_free(s->household);
+    dev_free(s->hh_off);
+    dev_free(s->wp_off);
+    dev_free(s->wp_people);
+    dev_free(s->hh_types);
+    dev_free(s->infected);
+    dev_free(s->staging);
+    dev_free(s->keys);
+    dev_free(s->er_people);
+    dev_free(s->er_off);
+    dev_free(s->claim_p);
+    dev_free(s->claim_s);
+    dev_free(s->list_p);
+    dev_free(s->list_s);
+    dev_free(s->blk_counts);
+    dev_free(s->blk_ninf);
+    dev_free(s->blk_off);
+    table_scratch_free(s->tables);
+    s->setup_done = s->seeded = s->have_snapshot = false;
+}
+
+AgentArrays agent_arrays(pt_sim* s, int32_t* trace_victim, uint8_t* trace_type) {
+    AgentArrays A;
+    A.age = s->age;
+    A.household = s->household;
+    A.hh_off = s->hh_off;
+    A.wp_off = s->wp_off;
+    A.wp_people = s->wp_people;
+    A.er_off = s->er_off;
+    A.er_people = s->er_people;
+    A.st_p = s->st_p;
+    A.st_s = s->st_s;
+    A.dy_p = s->dy_p;
+    A.dy_s = s->dy_s;
+    A.gn_p = s->gn_p;
+    A.gn_s = s->gn_s;
+    A.claim_p = s->claim_p;
+    A.claim_s = s->claim_s;
+    A.list_p = s->list_p;
+    A.list_s = s->list_s;
+    A.trace_victim = trace_victim;
+    A.trace_type = trace_type;
+    return A;
+}
+
+// Enqueues one loop iteration for host day index d (runToCompletion body,
+// PandemicSim.cu:882-954).  Every kernel is a no-op once the device state
+// says the loop has ended.
+int enqueue_day(pt_sim* s, int d) {
+    const bool weekend = (d % 7) >= 5;
+    hipStream_t stc = s->stream;
+    LaunchTimer* tm = &s->timer;
+    const int N = (int)s->N;
+    {
+        Scoped t(tm, KC_SWEEP);
+        k_sweep<<<s->nblk_sweep, kSweepThreads, 0, stc>>>(s->N, s->d_state, s->st_p, s->st_s, s->dy_p, s->dy_s,
+                                                          s->blk_counts, s->blk_ninf, s->staging);
+    }
+    {
+        Scoped t(tm, KC_PLAN);
+        k_plan<<<1, kPlanThreads, 0, stc>>>(s->d_state, s->nblk_sweep, s->blk_counts, s->blk_ninf, s->blk_off,
+                                            s->d_counts, s->d_inf, s->N);
+    }
+    {
+        Scoped t(tm, KC_COMPACT);
+        k_compact<<<s->nblk_sweep, 256, 0, stc>>>(s->d_state, s->staging, s->blk_ninf, s->blk_off, s->infected);
+    }
+    int64_t n;
+    int32_t K;
+    IdMode mode;
+    if (weekend) {
+        Scoped t(tm, KC_GEN_WEEKEND);
+        k_gen_weekend<<<s->grid_gen, 256, 0, stc>>>(s->d_model, s->d_state, s->keys);
+        n = 3 * s->N;
+        K = kWeHours * s->model.we_E;
+        mode = ID_DIV3;
+    } else {
+        Scoped t(tm, KC_GEN_WEEKDAY);
+        k_gen_weekday<<<s->grid_gen, 256, 0, stc>>>(s->d_model, s->d_state, s->age, s->keys);
+        n = 2 * s->N;
+        K = kWdHours * s->model.wd_E;
+        mode = ID_HOUR_MAJOR;
+    }
+    hipError_t e = build_table(s->tables, s->keys, n, K, mode, N, s->er_people, s->er_off, &s->d_state->done, stc, tm);
+    if (e != hipSuccess) return fail(PT_EDEVICE, "table build: %s", hipGetErrorString(e));
+    AgentArrays A = agent_arrays(s, nullptr, nullptr);
+    if (weekend) {
+        Scoped t(tm, KC_CONTACTS_WEEKEND);
+        k_contacts_weekend<<<s->grid_contacts, kContactThreads, 0, stc>>>(s->d_model, s->d_state, s->infected, A);
+    } else {
+        Scoped t(tm, KC_CONTACTS_WEEKDAY);
+        k_contacts_weekday<<<s->grid_contacts, kContactThreads, 0, stc>>>(s->d_model, s->d_state, s->infected, A,
+                                                                           s->workplace_off);
+    }
+    {
+        Scoped t(tm, KC_APPLY);
+        k_apply<<<s->grid_contacts, 256, 0, stc>>>(s->d_state, s->claim_p, s->claim_s, s->list_p, s->list_s, s->st_p,
+                                                    s->st_s, s->dy_p, s->dy_s, s->gn_p, s->gn_s);
+    }
+    {
+        Scoped t(tm, KC_END_DAY);
+        k_end_day<<<1, 1, 0, stc>>>(s->d_state);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail(PT_EDEVICE, "day launch: %s", hipGetErrorString(e));
+    s->last_weekend = weekend ? 1 : 0;
+    s->last_table_n = n;
+    s->host_day = d + 1;
+    return 0;
+}
+
+int reset_day_state(pt_sim* s) {
+    DayState h{};
+    h.day = 0;
+    h.done = s->params.max_days <= 0 ? 1 : 0;
+    h.rand_offset = s->rand_after_setup;
+    h.max_days = s->params.max_days;
+    HIP_TRY(hipMemcpyAsync(s->d_state, &h, sizeof h, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->host_day = 0;
+    s->last_weekend = -1;
+    return 0;
+}
+
+int read_state(pt_sim* s, DayState* h) {
+    HIP_TRY(hipMemcpyAsync(h, s->d_state, sizeof *h, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int grid_for(int64_t n, int threads, int cap) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>((n + threads - 1) / threads, cap));
+}
+
+}  // namespace
+
+// ============================ C ABI ============================
+extern "C" {
+
+const char* pt_last_error(void) { return g_err.c_str(); }
+
+int pt_glibc_seeds(int32_t core_seed, int32_t seeds_out[4]) {
+    if (!seeds_out) return fail(PT_EINVAL, "null output");
+    srand((unsigned)core_seed);
+    for (int i = 0; i < 4; ++i) seeds_out[i] = rand();
+    return PT_OK;
+}
+
+int pt_glibc_initial_infected(int32_t core_seed, int32_t n_people, int32_t n, int32_t* ids_out) {
+    if (n < 0 || n > n_people || (n > 0 && !ids_out)) return fail(PT_EINVAL, "initial infected exceeds population");
+    srand((unsigned)core_seed);
+    for (int i = 0; i < 4; ++i) (void)rand();
+    for (int i = 0; i < n; ++i) {
+        bool dup;
+        do {
+            ids_out[i] = rand() % n_people;
+            dup = false;
+            for (int j = 0; j < i; ++j)
+                if (ids_out[j] == ids_out[i]) {
+                    dup = true;
+                    break;
+                }
+        } while (dup);
+    }
+    return PT_OK;
+}
+
+int pt_sim_create(const pt_params* p, const int32_t seed_key[2], int32_t device, pt_sim** out) {
+    if (!p || !seed_key || !out) return fail(PT_EINVAL, "null argument");
+    *out = nullptr;
+    if (p->max_days < 0 || p->max_days > 255)
+        return fail(PT_EINVAL, "max_days %d outside [0,255] (day_t is 8 bits)", p->max_days);
+    if (!(p->people_scaling > 0.f) || !(p->location_scaling > 0.f))
+        return fail(PT_EINVAL, "scaling factors must be positive");
+    if (p->init_pandemic < 0 || p->init_seasonal < 0) return fail(PT_EINVAL, "negative initial infected");
+    const double hh = (double)p->people_scaling * (double)1000000;
+    if (hh > 2.0e8) return fail(PT_EINVAL, "population scale %g too large", (double)p->people_scaling);
+    if ((double)p->location_scaling > 1.0e4) return fail(PT_EINVAL, "location scale too large");
+    HIP_TRY(hipSetDevice(device));
+    pt_sim* s = new pt_sim();
+    s->device = device;
+    s->params = *p;
+    Model& m = s->model;
+    base_tables(m);
+    // setup_scaleSimulation (PandemicSim.cu:1127-1162)
+    s->H = round_half_up(hh);
+    int sum = 0;
+    for (int t = 0; t < kTypes; ++t) {
+        const int orig = round_half_up((double)kBaseCounts[t]);
+        const float prod = p->location_scaling * (float)orig;  // float * int
+        int nc = round_half_up((double)prod);
+        if (nc == 0 && orig > 0) nc = 1;
+        m.wp_count[t] = nc;
+        sum += nc;
+    }
+    s->W = sum;
+    int acc = 0;
+    for (int t = 0; t < kTypes; ++t) {
+        m.wp_offset[t] = acc;
+        acc += m.wp_count[t];
+    }
+    // setup_calculateInfectionData (PandemicSim.cu:2828-2840)
+    for (int i = 3; i < 6; ++i)
+        for (int j = 0; j < 10; ++j) m.shed[i][j] = m.shed[i][j] * p->asymp_factor;
+    const float r[2] = {p->r_pandemic, p->r_seasonal};
+    for (int i = 0; i < 2; ++i) {
+        const float d = (1.0f - p->asymp_factor) * p->percent_symptomatic;
+        m.factor[i] = r[i] / d;
+    }
+    m.pct = p->percent_symptomatic;
+    m.key0 = (uint64_t)(int64_t)seed_key[0];
+    m.key1 = (uint64_t)(int64_t)seed_key[1];
+    m.H = s->H;
+    m.W = s->W;
+    m.wd_base = m.wp_offset[kTypeAfterschool];
+    m.wd_E = s->W - m.wd_base;
+    m.we_base = m.wp_offset[kFirstErrandRow];
+    m.we_E = s->W - m.we_base;
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(PT_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    s->timer.stream = s->stream;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0) {
+        s->grid_contacts = cus * 8;
+        s->grid_gen = cus * 16;
+    }
+    if (dev_alloc(&s->d_model, 1) || dev_alloc(&s->d_state, 1) || dev_alloc(&s->d_counts, 256 * 16) ||
+        dev_alloc(&s->d_inf, 256) || dev_alloc(&s->d_scratch64, 512)) {
+        const std::string msg = g_err;
+        pt_sim_destroy(s);
+        return fail(PT_ENOMEM, "%s", msg.c_str());
+    }
+    (void)hipEventCreate(&s->run_begin);
+    (void)hipEventCreate(&s->run_end);
+    *out = s;
+    return PT_OK;
+}
+
+int pt_sim_setup(pt_sim* s, int64_t* n_people_out) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    HIP_TRY(hipSetDevice(s->device));
+    free_population(s);
+    Model& m = s->model;
+    const int H = s->H;
+    hipStream_t st = s->stream;
+    // K1/K2: household types and the population size (PandemicSim.cu:3634-3651, :2552-2571).
+    if (dev_alloc(&s->hh_types, (int64_t)H + 1)) return PT_ENOMEM;
+    HIP_TRY(hipMemsetAsync(s->hh_types, 0, (size_t)H + 1, st));
+    HIP_TRY(hipMemsetAsync(s->d_scratch64, 0, sizeof(unsigned long long), st));
+    m.N = 0;
+    HIP_TRY(hipMemcpyAsync(s->d_model, &m, sizeof m, hipMemcpyHostToDevice, st));
+    k_household_types<<<grid_for(H / 4 + 1, 256, 65535), 256, 0, st>>>(s->d_model, s->hh_types, s->d_scratch64);
+    unsigned long long npeople = 0;
+    HIP_TRY(hipMemcpyAsync(&npeople, s->d_scratch64, sizeof npeople, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (npeople == 0 || npeople > (unsigned long long)(0x7FFFFFFF / 8))
+        return fail(PT_EINVAL, "population %llu outside the supported range", npeople);
+    const int64_t N = (int64_t)npeople;
+    if ((int64_t)s->params.init_pandemic + s->params.init_seasonal > N)
+        return fail(PT_EINVAL, "initial infected exceed the population");
+    s->N = N;
+    m.N = (int32_t)N;
+    HIP_TRY(hipMemcpyAsync(s->d_model, &m, sizeof m, hipMemcpyHostToDevice, st));
+    // Per-agent arrays (setup_sizeGlobalArrays, PandemicSim.cu:1047-1107).
+    s->nblk_sweep = (int)((N + kSweepTile - 1) / kSweepTile);
+    const int64_t kmax = std::max<int64_t>((int64_t)kWdHours * m.wd_E, (int64_t)kWeHours * m.we_E);
+    const int64_t n3 = 3 * N;
+    const int64_t Npad = ((N + 15) / 16) * 16;  // the sweep reads whole uint4 words
+    if (dev_alloc(&s->st_p, Npad) || dev_alloc(&s->st_s, Npad) || dev_alloc(&s->dy_p, Npad) ||
+        dev_alloc(&s->dy_s, Npad) || dev_alloc(&s->gn_p, N) || dev_alloc(&s->gn_s, N) || dev_alloc(&s->age, N) ||
+        dev_alloc(&s->snap, 6 * N) || dev_alloc(&s->household, N) || dev_alloc(&s->hh_off, (int64_t)H + 1) ||
+        dev_alloc(&s->wp_off, (int64_t)s->W + 1) || dev_alloc(&s->wp_people, N) || dev_alloc(&s->infected, N) ||
+        dev_alloc(&s->staging, (int64_t)s->nblk_sweep * kSweepTile) || dev_alloc(&s->keys, n3) ||
+        dev_alloc(&s->er_people, n3) || dev_alloc(&s->er_off, std::max<int64_t>(kmax, (int64_t)s->W) + 1) ||
+        dev_alloc(&s->claim_p, N) || dev_alloc(&s->claim_s, N) || dev_alloc(&s->list_p, N) ||
+        dev_alloc(&s->list_s, N) || dev_alloc(&s->blk_counts, (int64_t)s->nblk_sweep * 16) ||
+        dev_alloc(&s->blk_ninf, s->nblk_sweep) || dev_alloc(&s->blk_off, s->nblk_sweep)) {
+        const std::string msg = g_err;
+        free_population(s);
+        return fail(PT_ENOMEM, "%s", msg.c_str());
+    }
+    if (table_scratch_alloc(s->tables, n3) != 0) {
+        free_population(s);
+        return fail(PT_ENOMEM, "table scratch allocation failed");
+    }
+    // K3/K4: household offsets by exclusive scan, then people
+    // (PandemicSim.cu:2573-2613).  keys doubles as the size array here.
+    uint32_t* sizes = s->keys;
+    k_household_sizes<<<grid_for((int64_t)H + 1, 256, 65535), 256, 0, st>>>(s->d_model, s->hh_types, sizes);
+    HIP_TRY(exclusive_scan_u32(sizes, (uint32_t*)s->hh_off, (int64_t)H + 1, s->tables.scan_part, nullptr, st));
+    k_generate_households<<<grid_for(H, 256, 65535), 256, 0, st>>>(s->d_model, s->hh_types, s->hh_off, s->household,
+                                                                   s->age);
+    uint64_t R = (uint64_t)(H / 4);
+    // K5/K6: workplaces, then the workplace table (PandemicSim.cu:3812-3862).
+    s->workplace_off = R;
+    k_assign_workplaces<<<grid_for(N / 4 + 1, 256, 65535), 256, 0, st>>>(s->d_model, s->workplace_off, s->age,
+                                                                         s->keys);
+    R += (uint64_t)(N / 4);
+    s->rand_after_setup = R;
+    hipError_t e = build_table(s->tables, s->keys, N, s->W, ID_IDENTITY, (int32_t)N, s->wp_people, s->wp_off, nullptr,
+                               st, nullptr);
+    if (e != hipSuccess) return fail(PT_EDEVICE, "workplace table: %s", hipGetErrorString(e));
+    // K7: statuses; claim words start empty.
+    k_init_status<<<grid_for(N, 256, 65535), 256, 0, st>>>(N, s->st_p, s->st_s, s->dy_p, s->dy_s, s->gn_p, s->gn_s);
+    k_fill_u64<<<grid_for(N, 256, 65535), 256, 0, st>>>(s->claim_p, N, ~0ull);
+    k_fill_u64<<<grid_for(N, 256, 65535), 256, 0, st>>>(s->claim_s, N, ~0ull);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    s->setup_done = true;
+    s->seeded = false;
+    if (reset_day_state(s)) return PT_EDEVICE;
+    if (n_people_out) *n_people_out = N;
+    return PT_OK;
+}
+
+int pt_sim_seed_infections(pt_sim* s, const int32_t* ids, int32_t n_p, int32_t n_s) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    if (!s->setup_done) return fail(PT_ESTATE, "seed_infections before setup");
+    if (n_p < 0 || n_s < 0 || (n_p + n_s > 0 && !ids)) return fail(PT_EINVAL, "bad initial infected");
+    const int n = n_p + n_s;
+    for (int i = 0; i < n; ++i)
+        if (ids[i] < 0 || ids[i] >= s->N) return fail(PT_EINVAL, "initial infected id %d out of range", ids[i]);
+    {
+        std::vector<int32_t> sorted(ids, ids + n);
+        std::sort(sorted.begin(), sorted.end());
+        for (int i = 1; i < n; ++i)
+            if (sorted[i] == sorted[i - 1]) return fail(PT_EINVAL, "duplicate initial infected id %d", sorted[i]);
+    }
+    HIP_TRY(hipSetDevice(s->device));
+    if (n > 0) {
+        int32_t* d_ids = nullptr;
+        if (dev_alloc(&d_ids, n)) return PT_ENOMEM;
+        HIP_TRY(hipMemcpyAsync(d_ids, ids, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+        k_seed_infections<<<(n + 255) / 256, 256, 0, s->stream>>>(d_ids, n_p, n_s, s->st_p, s->st_s, s->dy_p, s->dy_s,
+                                                                  s->gn_p, s->gn_s);
+        hipError_t e = hipStreamSynchronize(s->stream);
+        (void)hipFree(d_ids);
+        if (e != hipSuccess) return fail(PT_EDEVICE, "seed infections: %s", hipGetErrorString(e));
+    }
+    s->seeded = true;
+    return reset_day_state(s);
+}
+
+int pt_sim_step(pt_sim* s, int32_t counts16_out[16], int32_t* infected_out) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    if (!s->setup_done) return fail(PT_ESTATE, "step before setup");
+    HIP_TRY(hipSetDevice(s->device));
+    DayState h;
+    if (read_state(s, &h)) return PT_EDEVICE;
+    if (h.done) return 1;
+    const int day = h.day;
+    if (enqueue_day(s, day)) return PT_EDEVICE;
+    if (read_state(s, &h)) return PT_EDEVICE;
+    s->timer.collect();
+    if (h.fault) return fail(PT_EINVAL, "a kernel saw an errand destination outside the model's range");
+    int32_t c[16], inf = 0;
+    HIP_TRY(hipMemcpy(c, s->d_counts + 16 * day, sizeof c, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&inf, s->d_inf + day, sizeof inf, hipMemcpyDeviceToHost));
+    if (counts16_out) memcpy(counts16_out, c, sizeof c);
+    if (infected_out) *infected_out = inf;
+    return h.done ? 1 : 0;
+}
+
+int pt_sim_run(pt_sim* s, int32_t* counts_out, int32_t* infected_out, int32_t* days_out) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    if (!s->setup_done) return fail(PT_ESTATE, "run before setup");
+    HIP_TRY(hipSetDevice(s->device));
+    DayState h;
+    if (read_state(s, &h)) return PT_EDEVICE;
+    const int first = h.day, first_rows = h.days_run;
+    HIP_TRY(hipEventRecord(s->run_begin, s->stream));
+    if (!h.done)
+        for (int d = first; d < s->params.max_days; ++d)
+            if (enqueue_day(s, d)) return PT_EDEVICE;
+    HIP_TRY(hipEventRecord(s->run_end, s->stream));
+    HIP_TRY(hipEventSynchronize(s->run_end));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, s->run_begin, s->run_end));
+    s->last_run_ms = ms;
+    s->timer.collect();
+    if (read_state(s, &h)) return PT_EDEVICE;
+    if (h.fault) return fail(PT_EINVAL, "a kernel saw an errand destination outside the model's range");
+    const int rows = h.days_run;
+    if (counts_out && rows > first_rows)
+        HIP_TRY(hipMemcpy(counts_out, s->d_counts + 16 * first_rows, sizeof(int32_t) * 16 * (rows - first_rows),
+                          hipMemcpyDeviceToHost));
+    if (infected_out && rows > first_rows)
+        HIP_TRY(hipMemcpy(infected_out, s->d_inf + first_rows, sizeof(int32_t) * (rows - first_rows),
+                          hipMemcpyDeviceToHost));
+    if (days_out) *days_out = rows - first_rows;
+    return PT_OK;
+}
+
+int pt_sim_generation_offsets(pt_sim* s, int32_t* c_p, int32_t* c_s, int32_t len) {
+    if (!s || !c_p || !c_s) return fail(PT_EINVAL, "null argument");
+    if (!s->setup_done) return fail(PT_ESTATE, "generation offsets before setup");
+    const int md = s->params.max_days;
+    if (len < md + 1) return fail(PT_EINVAL, "len must be >= max_days + 1");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipMemsetAsync(s->d_scratch64, 0, 512 * sizeof(unsigned long long), s->stream));
+    k_gen_hist<<<grid_for(s->N, 256, 1024), 256, 0, s->stream>>>(s->gn_p, s->gn_s, s->N, s->d_scratch64);
+    unsigned long long hist[512];
+    HIP_TRY(hipMemcpyAsync(hist, s->d_scratch64, sizeof hist, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    int64_t ap = 0, as = 0;
+    for (int k = 0; k < md; ++k) {
+        c_p[k] = (int32_t)ap;
+        c_s[k] = (int32_t)as;
+        ap += (int64_t)hist[k];
+        as += (int64_t)hist[256 + k];
+    }
+    c_p[md] = (int32_t)s->N;
+    c_s[md] = (int32_t)s->N;
+    return PT_OK;
+}
+
+int64_t pt_sim_array_bytes(pt_sim* s, int32_t which) {
+    if (!s || !s->setup_done) return -1;
+    const int64_t N = s->N;
+    switch (which) {
+        case PT_STATUS_P: case PT_STATUS_S: case PT_DAY_P: case PT_DAY_S: case PT_GEN_P: case PT_GEN_S:
+        case PT_AGE: return N;
+        case PT_HOUSEHOLD: return 4 * N;
+        case PT_HH_OFFSETS: return 4 * ((int64_t)s->H + 1);
+        case PT_WP_OFFSETS: return 4 * ((int64_t)s->W + 1);
+        case PT_WP_PEOPLE: return 4 * N;
+        case PT_INFECTED: {
+            DayState h;
+            if (read_state(s, &h)) return -1;
+            return s->host_day > 0 ? 4 * (int64_t)h.n_infected : 0;
+        }
+        case PT_ERRAND_OFFSETS:
+            if (s->last_weekend < 0) return 0;
+            return 4 * ((int64_t)(s->last_weekend ? kWeHours : kWdHours) * s->W + 1);
+        case PT_ERRAND_PEOPLE: return s->last_weekend < 0 ? 0 : 4 * s->last_table_n;
+        case PT_HH_TYPES: return (int64_t)s->H + 1;
+    }
+    return -1;
+}
+
+int pt_sim_copy_array(pt_sim* s, int32_t which, void* dst, int64_t bytes) {
+    if (!s || !dst) return fail(PT_EINVAL, "null argument");
+    if (!s->setup_done) return fail(PT_ESTATE, "copy before setup");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const int64_t need = pt_sim_array_bytes(s, which);
+    if (need < 0) return fail(PT_EINVAL, "unknown array %d", which);
+    if (bytes < need) return fail(PT_EINVAL, "destination holds %lld bytes, %lld needed", (long long)bytes,
+                                  (long long)need);
+    if (need == 0) return PT_OK;
+    const void* src = nullptr;
+    switch (which) {
+        case PT_STATUS_P: src = s->st_p; break;
+        case PT_STATUS_S: src = s->st_s; break;
+        case PT_DAY_P: src = s->dy_p; break;
+        case PT_DAY_S: src = s->dy_s; break;
+        case PT_GEN_P: src = s->gn_p; break;
+        case PT_GEN_S: src = s->gn_s; break;
+        case PT_AGE: src = s->age; break;
+        case PT_HOUSEHOLD: src = s->household; break;
+        case PT_HH_OFFSETS: src = s->hh_off; break;
+        case PT_WP_OFFSETS: src = s->wp_off; break;
+        case PT_WP_PEOPLE: src = s->wp_people; break;
+        case PT_INFECTED: src = s->infected; break;
+        case PT_ERRAND_PEOPLE: src = s->er_people; break;
+        case PT_HH_TYPES: src = s->hh_types; break;
+        case PT_ERRAND_OFFSETS: {
+            // Expand the dense bucket layout to the reference's
+            // hour * W + location layout (lower_bound semantics).
+            const bool we = s->last_weekend == 1;
+            const int hours = we ? kWeHours : kWdHours;
+            const int E = we ? s->model.we_E : s->model.wd_E;
+            const int base = we ? s->model.we_base : s->model.wd_base;
+            std::vector<int32_t> dense((size_t)hours * E + 1);
+            HIP_TRY(hipMemcpy(dense.data(), s->er_off, dense.size() * 4, hipMemcpyDeviceToHost));
+            int32_t* out = (int32_t*)dst;
+            for (int hr = 0; hr < hours; ++hr)
+                for (int l = 0; l < s->W; ++l)
+                    out[(int64_t)hr * s->W + l] = l < base ? dense[(size_t)hr * E] : dense[(size_t)hr * E + (l - base)];
+            out[(int64_t)hours * s->W] = dense[(size_t)hours * E];
+            return PT_OK;
+        }
+        default: return fail(PT_EINVAL, "unknown array %d", which);
+    }
+    HIP_TRY(hipMemcpy(dst, src, (size_t)need, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+int pt_sim_get_scalars(pt_sim* s, int64_t out[8]) {
+    if (!s || !out) return fail(PT_EINVAL, "null argument");
+    DayState h{};
+    if (s->setup_done && read_state(s, &h)) return PT_EDEVICE;
+    out[0] = s->N;
+    out[1] = s->H;
+    out[2] = s->W;
+    out[3] = (int64_t)h.rand_offset;
+    out[4] = (int64_t)s->workplace_off;
+    out[5] = (int64_t)h.errand_off;
+    out[6] = h.day;
+    out[7] = h.days_run;
+    return PT_OK;
+}
+
+int pt_sim_snapshot(pt_sim* s) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    if (!s->setup_done) return fail(PT_ESTATE, "snapshot before setup");
+    HIP_TRY(hipSetDevice(s->device));
+    const int64_t N = s->N;
+    uint8_t* arr[6] = {s->st_p, s->st_s, s->dy_p, s->dy_s, s->gn_p, s->gn_s};
+    for (int i = 0; i < 6; ++i)
+        HIP_TRY(hipMemcpyAsync(s->snap + i * N, arr[i], (size_t)N, hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->have_snapshot = true;
+    return PT_OK;
+}
+
+int pt_sim_restore(pt_sim* s) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    if (!s->have_snapshot) return fail(PT_ESTATE, "restore without snapshot");
+    HIP_TRY(hipSetDevice(s->device));
+    const int64_t N = s->N;
+    uint8_t* arr[6] = {s->st_p, s->st_s, s->dy_p, s->dy_s, s->gn_p, s->gn_s};
+    for (int i = 0; i < 6; ++i)
+        HIP_TRY(hipMemcpyAsync(arr[i], s->snap + i * N, (size_t)N, hipMemcpyDeviceToDevice, s->stream));
+    k_fill_u64<<<grid_for(N, 256, 65535), 256, 0, s->stream>>>(s->claim_p, N, ~0ull);
+    k_fill_u64<<<grid_for(N, 256, 65535), 256, 0, s->stream>>>(s->claim_s, N, ~0ull);
+    HIP_TRY(hipGetLastError());
+    return reset_day_state(s);
+}
+
+int pt_sim_enable_timing(pt_sim* s, int32_t enable) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    s->timer.reset();
+    s->timer.on = enable != 0;
+    return PT_OK;
+}
+
+int pt_sim_kernel_times(pt_sim* s, double* ms_out, int64_t* launches_out, int32_t n) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    for (int k = 0; k < n && k < KC_COUNT; ++k) {
+        if (ms_out) ms_out[k] = s->timer.ms[k];
+        if (launches_out) launches_out[k] = s->timer.launches[k];
+    }
+    return KC_COUNT;
+}
+
+const char* pt_sim_kernel_name(int32_t k) { return (k >= 0 && k < KC_COUNT) ? kKernelNames[k] : ""; }
+
+int pt_sim_synchronize(pt_sim* s) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return PT_OK;
+}
+
+double pt_sim_last_run_ms(pt_sim* s) { return s ? s->last_run_ms : -1.0; }
+
+void pt_sim_destroy(pt_sim* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_population(s);
+    dev_free(s->d_model);
+    dev_free(s->d_state);
+    dev_free(s->d_counts);
+    dev_free(s->d_inf);
+    dev_free(s->d_scratch64);
+    if (s->run_begin) (void)hipEventDestroy(s->run_begin);
+    if (s->run_end) (void)hipEventDestroy(s->run_end);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+}  // extern "C"
+

@@ -1,0 +1,72 @@
+// pt_sort.h — stable location tables on gfx950.
+//
+// Replaces the reference's cub::DeviceRadixSort::SortPairs +
+// thrust::lower_bound pairs (PandemicSim.cu:1212-1248, :1316-1354,
+// :3835-3862).  Input: n uint32 bucket keys < K whose person ids are implicit
+// in the input position (ids ascend within each key in input order).
+// Output: people[n] (ids grouped by key, ascending within a key) and
+// offsets[K+1] with offsets[k] = number of keys < k.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pt {
+
+// Kernel classes timed with HIP events (pt_sim_kernel_times).
+enum KernelClass : int {
+    KC_SWEEP = 0,
+    KC_PLAN,
+    KC_COMPACT,
+    KC_GEN_WEEKDAY,
+    KC_GEN_WEEKEND,
+    KC_SORT_UPSWEEP,
+    KC_SORT_SCAN,
+    KC_SORT_DOWNSWEEP,
+    KC_OFFSETS,
+    KC_CONTACTS_WEEKDAY,
+    KC_CONTACTS_WEEKEND,
+    KC_APPLY,
+    KC_END_DAY,
+    KC_SETUP,
+    KC_COUNT
+};
+
+// Optional launch timer: begin/end bracket one launch on the stream.
+struct LaunchTimer {
+    virtual void begin(int cls) = 0;
+    virtual void end(int cls) = 0;
+    virtual ~LaunchTimer() {}
+};
+
+// How a person id follows from the input position j.
+enum IdMode : int {
+    ID_IDENTITY = 0,    // id = j                  (workplace table)
+    ID_HOUR_MAJOR = 1,  // id = j mod N            (weekday: hour 0 block, then hour 1 block)
+    ID_DIV3 = 2,        // id = j / 3              (weekend: 3 errands per person)
+};
+
+struct TableScratch {
+    uint32_t* keys_b = nullptr;  // n_cap
+    int32_t* vals_a = nullptr;   // n_cap
+    int32_t* vals_b = nullptr;   // n_cap
+    uint32_t* hist = nullptr;    // 256 * kMaxSortBlocks
+    uint32_t* scan_part = nullptr;
+    int64_t n_cap = 0;
+};
+
+int table_scratch_alloc(TableScratch& s, int64_t n_cap);
+void table_scratch_free(TableScratch& s);
+
+// Builds the table.  keys is clobbered (it ends up holding the sorted keys).
+// done (device int, may be null) turns every kernel into a no-op when set.
+hipError_t build_table(TableScratch& s, uint32_t* keys, int64_t n, int32_t K, IdMode mode, int32_t N,
+                       int32_t* people_out, int32_t* offsets_out, const int32_t* done, hipStream_t stream,
+                       LaunchTimer* tm);
+
+// Exclusive scan of n uint32 values (in and out may alias); part needs
+// ceil(n / 8192) + 1 entries.
+hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* part, const int32_t* done,
+                              hipStream_t stream);
+
+}  // namespace pt

@@ -1,0 +1,382 @@
+// pt_sort.hip — stable location tables: LSD radix sort of implicit-id pairs
+// with wave64 match ranking and an LDS-staged scatter, then bucket offsets.
+//
+// Replaces cub::DeviceRadixSort::SortPairs + thrust::lower_bound
+// (PandemicSim.cu:1212-1248, :1316-1354, :3835-3862).  Keys are dense bucket
+// ids, so only ceil(log2 K) bits are sorted (2-3 passes of <= 8 bits instead
+// of the reference's 7 passes over all 32 bits).
+#include "pt_sort.h"
+
+#include <algorithm>
+
+namespace pt {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kItems = 16;
+constexpr int kTile = kThreads * kItems;  // 4096 pairs per tile
+constexpr int kMaxSortBlocks = 1024;
+constexpr int kScanSeg = 8192;             // elements per scan block
+constexpr int kScanItems = kScanSeg / kThreads;
+
+__device__ __forceinline__ int32_t implicit_id(int64_t j, int mode, int32_t N) {
+    if (mode == ID_HOUR_MAJOR) return (int32_t)(j < N ? j : j - N);
+    if (mode == ID_DIV3) return (int32_t)(j / 3);
+    return (int32_t)j;
+}
+
+__device__ __forceinline__ bool is_done(const int32_t* done) { return done && *done; }
+
+// Inclusive scan across a wave64.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_w, uint32_t* total) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t inc = wave_incl_scan(v, lane);
+    if (lane == 63) s_w[wave] = inc;
+    __syncthreads();
+    uint32_t wbase = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+        const uint32_t x = s_w[w];
+        if (w < wave) wbase += x;
+        tot += x;
+    }
+    __syncthreads();
+    if (total) *total = tot;
+    return wbase + inc - v;
+}
+
+// Per-block digit histogram of the block's chunk.
+template <int BITS>
+__global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t* __restrict__ keys, int64_t n, int64_t chunk,
+                                                      int shift, uint32_t* __restrict__ hist, int nblk,
+                                                      const int32_t* done) {
+    if (is_done(done)) return;
+    constexpr int RADIX = 1 << BITS;
+    __shared__ uint32_t s_h[4][RADIX];
+    const int tid = threadIdx.x, wave = tid >> 6;
+    for (int i = tid; i < 4 * RADIX; i += kThreads) (&s_h[0][0])[i] = 0;
+    __syncthreads();
+    const int64_t begin = (int64_t)blockIdx.x * chunk;
+    const int64_t end = min(n, begin + chunk);
+    const uint4* k4 = reinterpret_cast<const uint4*>(keys);
+    for (int64_t q = begin / 4 + tid; q * 4 < end; q += kThreads) {
+        if (q * 4 + 3 < end) {
+            const uint4 v = k4[q];
+            atomicAdd(&s_h[wave][(v.x >> shift) & (RADIX - 1)], 1u);
+            atomicAdd(&s_h[wave][(v.y >> shift) & (RADIX - 1)], 1u);
+            atomicAdd(&s_h[wave][(v.z >> shift) & (RADIX - 1)], 1u);
+            atomicAdd(&s_h[wave][(v.w >> shift) & (RADIX - 1)], 1u);
+        } else {
+            for (int64_t j = q * 4; j < end; ++j) atomicAdd(&s_h[wave][(keys[j] >> shift) & (RADIX - 1)], 1u);
+        }
+    }
+    __syncthreads();
+    for (int d = tid; d < RADIX; d += kThreads)
+        hist[(int64_t)d * nblk + blockIdx.x] = s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d];
+}
+
+// Stable scatter of the block's chunk by one digit.  Each tile of 4096 pairs
+// is ranked with wave64 ballots (match-any on the digit bits), staged in LDS
+// in digit order and written out as contiguous runs per digit.
+template <int BITS>
+__global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restrict__ keys_in,
+                                                        const int32_t* __restrict__ vals_in, int id_mode, int32_t N,
+                                                        uint32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out,
+                                                        const uint32_t* __restrict__ hist, int64_t n, int64_t chunk,
+                                                        int shift, int nblk, const int32_t* done) {
+    if (is_done(done)) return;
+    constexpr int RADIX = 1 << BITS;
+    static_assert(RADIX <= kThreads, "one digit per thread in the digit scans");
+    constexpr uint32_t MASK = RADIX - 1;
+    __shared__ uint32_t s_keys[kTile];
+    __shared__ int32_t s_vals[kTile];
+    __shared__ uint32_t s_wcnt[4][RADIX];
+    __shared__ uint32_t s_start[RADIX];
+    __shared__ uint32_t s_tot[RADIX];
+    __shared__ uint32_t s_base[RADIX];
+    __shared__ uint32_t s_w[4];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t begin = (int64_t)blockIdx.x * chunk;
+    const int64_t end = min(n, begin + chunk);
+    if (tid < RADIX) s_base[tid] = hist[(int64_t)tid * nblk + blockIdx.x];
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+    for (int64_t t0 = begin; t0 < end; t0 += kTile) {
+        const int tile_n = (int)min((int64_t)kTile, end - t0);
+        for (int i = tid; i < 4 * RADIX; i += kThreads) (&s_wcnt[0][0])[i] = 0;
+        __syncthreads();
+        uint32_t k[kItems];
+        int32_t v[kItems];
+        uint32_t rk[kItems];
+#pragma unroll
+        for (int r = 0; r < kItems; ++r) {
+            const int e = wave * (kItems * 64) + r * 64 + lane;
+            const bool valid = e < tile_n;
+            const int64_t j = t0 + e;
+            k[r] = valid ? keys_in[j] : 0u;
+            v[r] = vals_in ? (valid ? vals_in[j] : 0) : implicit_id(j, id_mode, N);
+            const uint32_t d = (k[r] >> shift) & MASK;
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < BITS; ++b) {
+                const bool bit = (d >> b) & 1u;
+                const uint64_t m = __ballot(bit);
+                peers &= bit ? m : ~m;
+            }
+            const uint32_t old = s_wcnt[wave][d];
+            rk[r] = old + (uint32_t)__popcll(peers & below);
+            if (valid && (peers & below) == 0) s_wcnt[wave][d] = old + (uint32_t)__popcll(peers);
+        }
+        __syncthreads();
+        uint32_t tot = 0;
+        if (tid < RADIX) {
+            const uint32_t c0 = s_wcnt[0][tid], c1 = s_wcnt[1][tid], c2 = s_wcnt[2][tid], c3 = s_wcnt[3][tid];
+            s_wcnt[0][tid] = 0;
+            s_wcnt[1][tid] = c0;
+            s_wcnt[2][tid] = c0 + c1;
+            s_wcnt[3][tid] = c0 + c1 + c2;
+            tot = c0 + c1 + c2 + c3;
+            s_tot[tid] = tot;
+        }
+        const uint32_t st = block_excl_scan256(tot, s_w, nullptr);
+        if (tid < RADIX) s_start[tid] = st;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kItems; ++r) {
+            const int e = wave * (kItems * 64) + r * 64 + lane;
+            if (e < tile_n) {
+                const uint32_t d = (k[r] >> shift) & MASK;
+                const uint32_t pos = s_start[d] + s_wcnt[wave][d] + rk[r];
+                s_keys[pos] = k[r];
+                s_vals[pos] = v[r];
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < tile_n; i += kThreads) {
+            const uint32_t kk = s_keys[i];
+            const uint32_t d = (kk >> shift) & MASK;
+            const uint32_t g = s_base[d] + ((uint32_t)i - s_start[d]);
+            keys_out[g] = kk;
+            vals_out[g] = s_vals[i];
+        }
+        __syncthreads();
+        if (tid < RADIX) s_base[tid] += s_tot[tid];
+        __syncthreads();
+    }
+}
+
+// Scan, phase 1: per-segment sums.
+__global__ __launch_bounds__(kThreads) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
+                                                          uint32_t* __restrict__ part, const int32_t* done) {
+    if (is_done(done)) return;
+    __shared__ uint32_t s_w[4];
+    const int64_t base = (int64_t)blockIdx.x * kScanSeg;
+    uint32_t sum = 0;
+    for (int i = threadIdx.x; i < kScanSeg; i += kThreads) {
+        const int64_t j = base + i;
+        if (j < n) sum += in[j];
+    }
+    uint32_t tot;
+    block_excl_scan256(sum, s_w, &tot);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// Scan, phase 2: exclusive scan of the segment sums (one block).
+__global__ __launch_bounds__(kThreads) void k_scan_parts(uint32_t* part, int nparts, const int32_t* done) {
+    if (is_done(done)) return;
+    __shared__ uint32_t s_w[4];
+    uint32_t carry = 0;
+    for (int b0 = 0; b0 < nparts; b0 += kThreads) {
+        const int i = b0 + threadIdx.x;
+        const uint32_t v = i < nparts ? part[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan256(v, s_w, &tot);
+        if (i < nparts) part[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+// Scan, phase 3: exclusive scan inside each segment plus its offset.
+__global__ __launch_bounds__(kThreads) void k_scan_down(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                        int64_t n, const uint32_t* __restrict__ part,
+                                                        const int32_t* done) {
+    if (is_done(done)) return;
+    __shared__ uint32_t s_w[4];
+    const int64_t base = (int64_t)blockIdx.x * kScanSeg + (int64_t)threadIdx.x * kScanItems;
+    uint32_t x[kScanItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const int64_t j = base + i;
+        x[i] = j < n ? in[j] : 0u;
+        sum += x[i];
+    }
+    uint32_t run = block_excl_scan256(sum, s_w, nullptr) + part[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const int64_t j = base + i;
+        if (j < n) out[j] = run;
+        run += x[i];
+    }
+}
+
+// offsets[q] = first position whose key is >= q, for q in (keys[i-1], keys[i]].
+__global__ void k_offsets(const uint32_t* __restrict__ keys, int64_t n, int32_t* __restrict__ offsets,
+                          const int32_t* done) {
+    if (is_done(done)) return;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int64_t k = keys[i];
+        const int64_t kp = i == 0 ? -1 : (int64_t)keys[i - 1];
+        for (int64_t q = kp + 1; q <= k; ++q) offsets[q] = (int32_t)i;
+    }
+}
+
+// offsets[q] = n for q past the last key (and for all q when n == 0).
+__global__ void k_offsets_tail(const uint32_t* __restrict__ keys, int64_t n, int32_t K, int32_t* __restrict__ offsets,
+                               const int32_t* done) {
+    if (is_done(done)) return;
+    const int64_t first = n == 0 ? 0 : (int64_t)keys[n - 1] + 1;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = first + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q <= K; q += stride)
+        offsets[q] = (int32_t)n;
+}
+
+struct Scoped {
+    LaunchTimer* tm;
+    int cls;
+    Scoped(LaunchTimer* t, int c) : tm(t), cls(c) {
+        if (tm) tm->begin(cls);
+    }
+    ~Scoped() {
+        if (tm) tm->end(cls);
+    }
+};
+
+template <int BITS>
+hipError_t radix_pass(TableScratch& s, const uint32_t* kin, const int32_t* vin, int id_mode, int32_t N, uint32_t* kout,
+                      int32_t* vout, int64_t n, int shift, int nblk, int64_t chunk, const int32_t* done,
+                      hipStream_t st, LaunchTimer* tm) {
+    {
+        Scoped t(tm, KC_SORT_UPSWEEP);
+        k_upsweep<BITS><<<nblk, kThreads, 0, st>>>(kin, n, chunk, shift, s.hist, nblk, done);
+    }
+    {
+        Scoped t(tm, KC_SORT_SCAN);
+        hipError_t e = exclusive_scan_u32(s.hist, s.hist, (int64_t)nblk << BITS, s.scan_part, done, st);
+        if (e != hipSuccess) return e;
+    }
+    {
+        Scoped t(tm, KC_SORT_DOWNSWEEP);
+        k_downsweep<BITS><<<nblk, kThreads, 0, st>>>(kin, vin, id_mode, N, kout, vout, s.hist, n, chunk, shift, nblk,
+                                                     done);
+    }
+    return hipGetLastError();
+}
+
+hipError_t radix_pass_bits(int bits, TableScratch& s, const uint32_t* kin, const int32_t* vin, int id_mode, int32_t N,
+                           uint32_t* kout, int32_t* vout, int64_t n, int shift, int nblk, int64_t chunk,
+                           const int32_t* done, hipStream_t st, LaunchTimer* tm) {
+    switch (bits) {
+        case 1: return radix_pass<1>(s, kin, vin, id_mode, N, kout, vout, n, shift, nblk, chunk, done, st, tm);
+        case 2: return radix_pass<2>(s, kin, vin, id_mode, N, kout, vout, n, shift, nblk, chunk, done, st, tm);
+        case 3: return radix_pass<3>(s, kin, vin, id_mode, N, kout, vout, n, shift, nblk, chunk, done, st, tm);
+        case 4: return radix_pass<4>(s, kin, vin, id_mode, N, kout, vout, n, shift, nblk, chunk, done, st, tm);
+        case 5: return radix_pass<5>(s, kin, vin, id_mode, N, kout, vout, n, shift, nblk, chunk, done, st, tm);
+        case 6: return radix_pass<6>(s, kin, vin, id_mode, N, kout, vout, n, shift, nblk, chunk, done, st, tm);
+        case 7: return radix_pass<7>(s, kin, vin, id_mode, N, kout, vout, n, shift, nblk, chunk, done, st, tm);
+        case 8: return radix_pass<8>(s, kin, vin, id_mode, N, kout, vout, n, shift, nblk, chunk, done, st, tm);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+int table_scratch_alloc(TableScratch& s, int64_t n_cap) {
+    table_scratch_free(s);
+    const int64_t n = std::max<int64_t>(n_cap, 1);
+    if (hipMalloc(&s.keys_b, n * sizeof(uint32_t)) != hipSuccess) return -1;
+    if (hipMalloc(&s.vals_a, n * sizeof(int32_t)) != hipSuccess) return -1;
+    if (hipMalloc(&s.vals_b, n * sizeof(int32_t)) != hipSuccess) return -1;
+    if (hipMalloc(&s.hist, (size_t)256 * kMaxSortBlocks * sizeof(uint32_t)) != hipSuccess) return -1;
+    // Partial sums for scans of up to max(n_cap, 256 * kMaxSortBlocks) values.
+    const int64_t parts = (std::max<int64_t>(n, (int64_t)256 * kMaxSortBlocks) + kScanSeg - 1) / kScanSeg + 2;
+    if (hipMalloc(&s.scan_part, (size_t)parts * sizeof(uint32_t)) != hipSuccess) return -1;
+    s.n_cap = n_cap;
+    return 0;
+}
+
+void table_scratch_free(TableScratch& s) {
+    if (s.keys_b) (void)hipFree(s.keys_b);
+    if (s.vals_a) (void)hipFree(s.vals_a);
+    if (s.vals_b) (void)hipFree(s.vals_b);
+    if (s.hist) (void)hipFree(s.hist);
+    if (s.scan_part) (void)hipFree(s.scan_part);
+    s = TableScratch();
+}
+
+hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* part, const int32_t* done,
+                              hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const int nparts = (int)((n + kScanSeg - 1) / kScanSeg);
+    k_scan_reduce<<<nparts, kThreads, 0, stream>>>(in, n, part, done);
+    k_scan_parts<<<1, kThreads, 0, stream>>>(part, nparts, done);
+    k_scan_down<<<nparts, kThreads, 0, stream>>>(in, out, n, part, done);
+    return hipGetLastError();
+}
+
+hipError_t build_table(TableScratch& s, uint32_t* keys, int64_t n, int32_t K, IdMode mode, int32_t N,
+                       int32_t* people_out, int32_t* offsets_out, const int32_t* done, hipStream_t stream,
+                       LaunchTimer* tm) {
+    if (n > s.n_cap || K < 1 || n >= (int64_t)1 << 32) return hipErrorInvalidValue;
+    int bits = 0;
+    while (((int64_t)1 << bits) < K) bits++;
+    if (bits == 0) bits = 1;
+    const int passes = (bits + 7) / 8;
+    // Split the bits as evenly as possible over the passes.
+    int pass_bits[4] = {0, 0, 0, 0};
+    for (int p = 0; p < passes; ++p) pass_bits[p] = bits / passes + (p < bits % passes ? 1 : 0);
+    uint32_t* kin = keys;
+    if (n > 0) {
+        const int64_t tiles = (n + kTile - 1) / kTile;
+        const int64_t tiles_per_blk = (tiles + kMaxSortBlocks - 1) / kMaxSortBlocks;
+        const int nblk = (int)((tiles + tiles_per_blk - 1) / tiles_per_blk);
+        const int64_t chunk = tiles_per_blk * kTile;
+        uint32_t* kout = s.keys_b;
+        const int32_t* vin = nullptr;
+        int shift = 0;
+        for (int p = 0; p < passes; ++p) {
+            int32_t* vout = (p == passes - 1) ? people_out : ((p & 1) ? s.vals_b : s.vals_a);
+            hipError_t e = radix_pass_bits(pass_bits[p], s, kin, vin, mode, N, kout, vout, n, shift, nblk, chunk, done,
+                                           stream, tm);
+            if (e != hipSuccess) return e;
+            shift += pass_bits[p];
+            vin = vout;
+            std::swap(kin, kout);
+        }
+    }
+    {
+        // kin now holds the sorted keys (the caller's buffer or keys_b).
+        Scoped t(tm, KC_OFFSETS);
+        if (n > 0) {
+            const int blocks = (int)std::min<int64_t>((n + kThreads - 1) / kThreads, 4096);
+            k_offsets<<<blocks, kThreads, 0, stream>>>(kin, n, offsets_out, done);
+        }
+        k_offsets_tail<<<64, kThreads, 0, stream>>>(kin, n, K, offsets_out, done);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace pt
