@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Benchmark of the daily simulation loop on one MI355X.
+
+Workload: 50,000,000-agent population x 100 days, the reference's
+constants.csv row `constants,100,1.8,1.3,10,10,21.24,21.24,1,0` with seed 1
+(50,002,479 agents).  One step = one whole 100-day loop (runToCompletion) on
+the GPU, started from the post-setup state kept in HBM (restored on the device
+before each step; setup itself is input preparation and not timed).
+
+value = person-days per second over the timed steps
+      = steps * N * days_simulated / wall seconds of those steps.
+
+roofline: the dominant kernel class by HIP-event time; `achieved` is its
+algorithmic bytes (each logical array read or written once, at its stored
+width) divided by its launch time, both summed over the timed steps.
+
+cpu_baseline: the CPU oracle (a sequential port of the reference semantics)
+on one host core, timed on a bounded sample of the same workload: setup plus
+the first CPU_DAYS days (setup excluded from the rate).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "person-days/sec at 50M×100d; batched vs plain replicates/sec; HBM GB/s"
+ROW = ["100", "1.8", "1.3", "10", "10", "21.24", "21.24", "1", "0"]
+SEED = 1
+HBM_PEAK_GBS = 8000.0
+CPU_DAYS = 4
+CONSTANTS_HEADER = ("constants,max_days,reproduction_number_pandemic,reproduction_number_seasonal,"
+                    "initial_infected_pandemic,initial_infected_seasonal,people_scaling_factor,"
+                    "location_scaling_factor,percent_symptomatic,asymp_factor")
+
+
+def algorithmic_bytes(N, W, lay, inf, counts):
+    """Algorithmic bytes per kernel class for one loop, and the loop total of
+    the per-day model B_wd = 43N + 89I + 3X + 8W + 4, B_we = 62N + 62I + 3X +
+    40W + 4 (terminal I = 0 day: 2N)."""
+    per = {}
+
+    def add(k, b):
+        per[k] = per.get(k, 0.0) + float(b)
+
+    loop = 0.0
+    days = len(inf)
+    for d in range(days):
+        I = int(inf[d])
+        add("sweep", 2 * N + 2 * I + 4 * I)          # statuses, days of the infected, staged ids
+        add("plan", 0)
+        if I == 0:
+            loop += 2 * N
+            continue
+        add("compact", 8 * I)
+        weekend = (d % 7) >= 5
+        if d + 1 < days:
+            X = (int(counts[d][1]) - int(counts[d + 1][1])) + (int(counts[d][9]) - int(counts[d + 1][9]))
+        else:
+            X = 0
+        if weekend:
+            n = 3 * N
+            passes = lay["we_passes"]
+            add("gen_weekend", 4 * n)
+            add("contacts_weekend", 56 * I)
+            loop += 62 * N + 62 * I + 3 * X + 40 * W + 4
+            buckets = lay["we_buckets"]
+        else:
+            n = 2 * N
+            passes = lay["wd_passes"]
+            add("gen_weekday", N + 4 * n)
+            add("contacts_weekday", 83 * I)
+            loop += 43 * N + 89 * I + 3 * X + 8 * W + 4
+            buckets = lay["wd_buckets"]
+        for p in range(passes):
+            add("sort_upsweep", 4 * n)
+            add("sort_downsweep", n * (4 + (4 if p else 0) + 4 + 4))
+        add("offsets", 4 * n + 4 * (buckets + 1))
+        add("apply", 3 * X + 16 * X)
+    return per, loop
+
+
+def cpu_baseline():
+    exe = os.path.join(ROOT, "oracle", "build", "pt_oracle")
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "constants.csv"), "w") as f:
+            f.write(CONSTANTS_HEADER + "\nconstants," + ",".join(ROW) + "\n")
+        with open(os.path.join(d, "seed.txt"), "w") as f:
+            f.write("%d\n" % SEED)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        out = subprocess.run([exe, "--time", str(CPU_DAYS)], cwd=d, env=env, stdout=subprocess.PIPE, text=True,
+                             timeout=600, check=True).stdout
+    r = json.loads(out.strip().splitlines()[-1])
+    return {"value": round(r["person_days_per_s"], 1), "unit": "person-days/s", "cores": 1, "kind": "port",
+            "sample": "oracle setup (%.1f s, untimed) + first %d days of the 50M x 100d workload "
+                      "(%d agents, %.1f s loop)" % (r["setup_s"], r["days"], r["n_people"], r["loop_s"])}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the daily loop of one simulation runs on a single MI355X "
+              "(multi-GPU sharding is a design note, not part of this build); use --gpus 1", file=sys.stderr)
+        return 2
+    if int(os.environ.get("RANK", "0")) != 0:
+        return 0
+
+    from pandemicthrust_amd import Params, PandemicSim
+    sim = PandemicSim(Params.from_row(ROW), SEED)
+    t0 = time.perf_counter()
+    N = sim.setup_sim()
+    setup_s = time.perf_counter() - t0
+    sim.snapshot()
+    lay = sim.layout()
+    W = int(sim.scalars()[2])
+    print("setup: %d agents in %.2f s" % (N, setup_s), file=sys.stderr, flush=True)
+
+    for i in range(args.warmup):
+        sim.restore()
+        sim.run()
+        print("warmup %d done" % i, file=sys.stderr, flush=True)
+
+    sim.enable_timing(True)
+    sim.synchronize()
+    days_total = 0
+    loop_ms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        sim.restore()
+        counts, inf = sim.run()
+        days_total += len(inf)
+        loop_ms.append(sim.last_run_ms())
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    kt = sim.kernel_times()
+    sim.enable_timing(False)
+    print("steps: loop ms %s" % ["%.1f" % x for x in loop_ms], file=sys.stderr, flush=True)
+
+    days_per_step = days_total / max(args.steps, 1)
+    value = N * days_total / wall
+    per_loop, loop_bytes = algorithmic_bytes(N, W, lay, inf, counts)
+
+    table = {}
+    for name, (ms, launches) in kt.items():
+        if launches == 0 or name == "setup":
+            continue
+        b = per_loop.get(name, 0.0) * args.steps
+        table[name] = {"ms": round(ms, 3), "launches": launches, "avg_us": round(1000 * ms / launches, 2),
+                       "GBps": round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+    dom = max(table, key=lambda k: table[k]["ms"])
+    dom_ms, dom_launches = kt[dom]
+    dom_bytes = per_loop.get(dom, 0.0) * args.steps
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    if args.kernels:
+        for k, v in sorted(table.items(), key=lambda kv: -kv[1]["ms"]):
+            print("%-18s %s" % (k, v), file=sys.stderr)
+
+    loop_s = sum(loop_ms) / 1000.0
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "person-days/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000 * wall / max(args.steps, 1), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8/i32 state, f32 draws",
+        "data": "synthetic (the reference's own generator: households, workplaces, errands from Threefry draws)",
+        "config": {"workload": "50M agents x 100 days: constants,100,1.8,1.3,10,10,21.24,21.24,1,0 seed 1",
+                   "n_people": N, "days_per_step": days_per_step, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_us": round(1000 * dom_ms / dom_launches, 2),
+                     "algorithmic_bytes_per_launch": round(dom_bytes / dom_launches, 1)},
+        "loop_roofline": {"algorithmic_GB_per_loop": round(loop_bytes / 1e9, 3),
+                          "achieved": round(loop_bytes * args.steps / loop_s / 1e9, 1),
+                          "frac": round(loop_bytes * args.steps / loop_s / 1e9 / HBM_PEAK_GBS, 4)},
+        "kernels": table,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+    sim.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

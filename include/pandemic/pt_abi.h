@@ -108,6 +108,13 @@ int pt_sim_copy_array(pt_sim* sim, int32_t which, void* dst, int64_t bytes);
  * [5] errand rand offset of the last day, [6] current day, [7] days run. */
 int pt_sim_get_scalars(pt_sim* sim, int64_t out[8]);
 
+/* Errand-table layout: [0] weekday base location, [1] weekday locations per
+ * hour E_wd, [2] weekend base location, [3] weekend locations per hour E_we,
+ * [4] weekday buckets (2 E_wd), [5] weekend buckets (10 E_we), [6] radix
+ * passes of a weekday table, [7] radix passes of a weekend table.  Bucket
+ * (hour, location) is stored densely at hour * E + (location - base). */
+int pt_sim_get_layout(pt_sim* sim, int64_t out[8]);
+
 /* Saves the post-setup, post-seeding state on the device; restore brings
  * the simulation back to it (used to repeat the daily loop). */
 int pt_sim_snapshot(pt_sim* sim);

@@ -1551,6 +1551,20 @@ int pt_sim_get_scalars(pt_sim* s, int64_t out[8]) {
     return PT_OK;
 }
 
+int pt_sim_get_layout(pt_sim* s, int64_t out[8]) {
+    if (!s || !out) return fail(PT_EINVAL, "null argument");
+    const Model& m = s->model;
+    out[0] = m.wd_base;
+    out[1] = m.wd_E;
+    out[2] = m.we_base;
+    out[3] = m.we_E;
+    out[4] = (int64_t)kWdHours * m.wd_E;
+    out[5] = (int64_t)kWeHours * m.we_E;
+    out[6] = radix_passes((int32_t)out[4]);
+    out[7] = radix_passes((int32_t)out[5]);
+    return PT_OK;
+}
+
 int pt_sim_snapshot(pt_sim* s) {
     if (!s) return fail(PT_EINVAL, "null handle");
     if (!s->setup_done) return fail(PT_ESTATE, "snapshot before setup");

@@ -64,6 +64,9 @@ hipError_t build_table(TableScratch& s, uint32_t* keys, int64_t n, int32_t K, Id
                        int32_t* people_out, int32_t* offsets_out, const int32_t* done, hipStream_t stream,
                        LaunchTimer* tm);
 
+// Number of radix passes build_table uses for K buckets.
+int radix_passes(int32_t K);
+
 // Exclusive scan of n uint32 values (in and out may alias); part needs
 // ceil(n / 8192) + 1 entries.
 hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* part, const int32_t* done,

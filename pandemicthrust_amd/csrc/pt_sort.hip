@@ -337,14 +337,20 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint
     return hipGetLastError();
 }
 
+static int key_bits(int32_t K) {
+    int bits = 0;
+    while (((int64_t)1 << bits) < K) bits++;
+    return bits == 0 ? 1 : bits;
+}
+
+int radix_passes(int32_t K) { return (key_bits(K) + 7) / 8; }
+
 hipError_t build_table(TableScratch& s, uint32_t* keys, int64_t n, int32_t K, IdMode mode, int32_t N,
                        int32_t* people_out, int32_t* offsets_out, const int32_t* done, hipStream_t stream,
                        LaunchTimer* tm) {
     if (n > s.n_cap || K < 1 || n >= (int64_t)1 << 32) return hipErrorInvalidValue;
-    int bits = 0;
-    while (((int64_t)1 << bits) < K) bits++;
-    if (bits == 0) bits = 1;
-    const int passes = (bits + 7) / 8;
+    const int bits = key_bits(K);
+    const int passes = radix_passes(K);
     // Split the bits as evenly as possible over the passes.
     int pass_bits[4] = {0, 0, 0, 0};
     for (int p = 0; p < passes; ++p) pass_bits[p] = bits / passes + (p < bits % passes ? 1 : 0);
