@@ -148,6 +148,7 @@ def main():
     kt = sim.kernel_times()
     sim.enable_timing(False)
     print("steps: loop ms %s" % ["%.1f" % x for x in loop_ms], file=sys.stderr, flush=True)
+    print("infected per day: %s" % " ".join(str(int(x)) for x in inf[::5]), file=sys.stderr, flush=True)
 
     days_per_step = days_total / max(args.steps, 1)
     value = N * days_total / wall

@@ -210,6 +210,12 @@ __global__ void k_init_status(int64_t N, uint8_t* st_p, uint8_t* st_s, uint8_t* 
     }
 }
 
+// Plain copy of n16 16-byte words (snapshot / restore of the agent state).
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 __global__ void k_fill_u64(uint64_t* p, int64_t n, uint64_t v) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
@@ -1317,8 +1323,9 @@ int pt_sim_setup(pt_sim* s, int64_t* n_people_out) {
     const int64_t n3 = 3 * N;
     const int64_t Npad = ((N + 15) / 16) * 16;  // the sweep reads whole uint4 words
     if (dev_alloc(&s->st_p, Npad) || dev_alloc(&s->st_s, Npad) || dev_alloc(&s->dy_p, Npad) ||
-        dev_alloc(&s->dy_s, Npad) || dev_alloc(&s->gn_p, N) || dev_alloc(&s->gn_s, N) || dev_alloc(&s->age, N) ||
-        dev_alloc(&s->snap, 6 * N) || dev_alloc(&s->household, N) || dev_alloc(&s->hh_off, (int64_t)H + 1) ||
+        dev_alloc(&s->dy_s, Npad) || dev_alloc(&s->gn_p, Npad) || dev_alloc(&s->gn_s, Npad) ||
+        dev_alloc(&s->age, N) || dev_alloc(&s->snap, 6 * Npad) || dev_alloc(&s->household, N) ||
+        dev_alloc(&s->hh_off, (int64_t)H + 1) ||
         dev_alloc(&s->wp_off, (int64_t)s->W + 1) || dev_alloc(&s->wp_people, N) || dev_alloc(&s->infected, N) ||
         dev_alloc(&s->staging, (int64_t)s->nblk_sweep * kSweepTile) || dev_alloc(&s->keys, n3) ||
         dev_alloc(&s->er_people, n3) || dev_alloc(&s->er_off, std::max<int64_t>(kmax, (int64_t)s->W) + 1) ||
@@ -1569,10 +1576,12 @@ int pt_sim_snapshot(pt_sim* s) {
     if (!s) return fail(PT_EINVAL, "null handle");
     if (!s->setup_done) return fail(PT_ESTATE, "snapshot before setup");
     HIP_TRY(hipSetDevice(s->device));
-    const int64_t N = s->N;
+    const int64_t Npad = ((s->N + 15) / 16) * 16;
     uint8_t* arr[6] = {s->st_p, s->st_s, s->dy_p, s->dy_s, s->gn_p, s->gn_s};
     for (int i = 0; i < 6; ++i)
-        HIP_TRY(hipMemcpyAsync(s->snap + i * N, arr[i], (size_t)N, hipMemcpyDeviceToDevice, s->stream));
+        k_copy16<<<grid_for(Npad / 16, 256, 8192), 256, 0, s->stream>>>(
+            reinterpret_cast<const uint4*>(arr[i]), reinterpret_cast<uint4*>(s->snap + i * Npad), Npad / 16);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->have_snapshot = true;
     return PT_OK;
@@ -1582,10 +1591,11 @@ int pt_sim_restore(pt_sim* s) {
     if (!s) return fail(PT_EINVAL, "null handle");
     if (!s->have_snapshot) return fail(PT_ESTATE, "restore without snapshot");
     HIP_TRY(hipSetDevice(s->device));
-    const int64_t N = s->N;
+    const int64_t N = s->N, Npad = ((N + 15) / 16) * 16;
     uint8_t* arr[6] = {s->st_p, s->st_s, s->dy_p, s->dy_s, s->gn_p, s->gn_s};
     for (int i = 0; i < 6; ++i)
-        HIP_TRY(hipMemcpyAsync(arr[i], s->snap + i * N, (size_t)N, hipMemcpyDeviceToDevice, s->stream));
+        k_copy16<<<grid_for(Npad / 16, 256, 8192), 256, 0, s->stream>>>(
+            reinterpret_cast<const uint4*>(s->snap + i * Npad), reinterpret_cast<uint4*>(arr[i]), Npad / 16);
     k_fill_u64<<<grid_for(N, 256, 65535), 256, 0, s->stream>>>(s->claim_p, N, ~0ull);
     k_fill_u64<<<grid_for(N, 256, 65535), 256, 0, s->stream>>>(s->claim_s, N, ~0ull);
     HIP_TRY(hipGetLastError());
