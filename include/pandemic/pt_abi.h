@@ -74,8 +74,9 @@ int pt_sim_seed_infections(pt_sim* sim, const int32_t* ids, int32_t n_pandemic, 
  * recover, compact, then the weekday or weekend step.  Writes the day's 16
  * status counts (daily_writeInfectedStats order: pandemic codes 0..7, then
  * seasonal 0..7) and the infected count.  Returns 0 after a day with
- * contacts, 1 when the loop has ended (I == 0 day or max_days reached; the
- * I == 0 day still reports its counts), negative on error. */
+ * contacts, 1 for the terminal I == 0 day (its counts are still reported)
+ * or when the loop had already ended (nothing is written), negative on
+ * error. */
 int pt_sim_step(pt_sim* sim, int32_t counts16_out[16], int32_t* infected_out);
 
 /* Runs the remaining days of runToCompletion without per-day host
@@ -114,6 +115,12 @@ int pt_sim_get_scalars(pt_sim* sim, int64_t out[8]);
  * passes of a weekday table, [7] radix passes of a weekend table.  Bucket
  * (hour, location) is stored densely at hour * E + (location - base). */
 int pt_sim_get_layout(pt_sim* sim, int64_t out[8]);
+
+/* Errand-table strategy: 0 automatic (default), 1 build every bucket each
+ * day, 2 build only the buckets that day's infectors draw contacts from.
+ * Simulation results are identical in all modes; the PT_ERRAND_* parity
+ * hooks show complete tables only in mode 1. */
+int pt_sim_set_table_mode(pt_sim* sim, int32_t mode);
 
 /* Saves the post-setup, post-seeding state on the device; restore brings
  * the simulation back to it (used to repeat the daily loop). */

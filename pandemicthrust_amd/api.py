@@ -22,7 +22,8 @@ _U8_ARRAYS = {PT_STATUS_P, PT_STATUS_S, PT_DAY_P, PT_DAY_S, PT_GEN_P, PT_GEN_S, 
 EXPORTS = [
     "pt_last_error", "pt_glibc_seeds", "pt_glibc_initial_infected", "pt_sim_create", "pt_sim_setup",
     "pt_sim_seed_infections", "pt_sim_step", "pt_sim_run", "pt_sim_generation_offsets", "pt_sim_array_bytes",
-    "pt_sim_copy_array", "pt_sim_get_scalars", "pt_sim_get_layout", "pt_sim_snapshot", "pt_sim_restore", "pt_sim_enable_timing",
+    "pt_sim_copy_array", "pt_sim_get_scalars", "pt_sim_get_layout", "pt_sim_set_table_mode", "pt_sim_snapshot",
+    "pt_sim_restore", "pt_sim_enable_timing",
     "pt_sim_kernel_times", "pt_sim_kernel_name", "pt_sim_synchronize", "pt_sim_last_run_ms", "pt_sim_destroy",
 ]
 
@@ -76,6 +77,7 @@ def lib():
         L.pt_sim_copy_array.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]
         L.pt_sim_get_scalars.argtypes = [C.c_void_p, C.c_void_p]
         L.pt_sim_get_layout.argtypes = [C.c_void_p, C.c_void_p]
+        L.pt_sim_set_table_mode.argtypes = [C.c_void_p, C.c_int32]
         L.pt_sim_snapshot.argtypes = [C.c_void_p]
         L.pt_sim_restore.argtypes = [C.c_void_p]
         L.pt_sim_enable_timing.argtypes = [C.c_void_p, C.c_int32]
@@ -182,6 +184,10 @@ class PandemicSim:
         _check(lib().pt_sim_get_layout(self.h, out.ctypes.data))
         keys = ["wd_base", "wd_E", "we_base", "we_E", "wd_buckets", "we_buckets", "wd_passes", "we_passes"]
         return dict(zip(keys, (int(x) for x in out)))
+
+    def set_table_mode(self, mode):
+        """0 automatic, 1 full errand tables every day, 2 queried buckets only."""
+        _check(lib().pt_sim_set_table_mode(self.h, mode))
 
     def snapshot(self):
         _check(lib().pt_sim_snapshot(self.h))

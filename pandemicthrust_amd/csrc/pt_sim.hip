@@ -77,7 +77,24 @@ struct DayState {
     uint32_t n_claimed[2]; // claim-list lengths (pandemic, seasonal)
     int32_t max_days;
     int32_t fault;         // nonzero if a kernel saw data outside the model's range
+    int32_t skip_full;     // the full errand-table path is idle today
+    int32_t skip_subset;   // the subset errand-table path is idle today
+    uint32_t n_sel;        // pairs selected into today's subset table
+    uint32_t n_marked;     // errand buckets queried today
+    int32_t skip_mark;     // too many infectors today for a subset table to pay off
 };
+
+// Errand tables are built either for every bucket (full) or only for the
+// buckets today's infectors draw contacts from (subset); the choice is
+// made on the device from the number of queried buckets.
+enum TableMode : int { TABLES_AUTO = 0, TABLES_FULL = 1, TABLES_SUBSET = 2 };
+constexpr int kSelThreads = 256;
+constexpr int kSelPerWd = 8;   // agents per thread, weekday selection
+constexpr int kSelPerWe = 4;   // agents per thread, weekend selection
+constexpr int kSelAgentsWd = kSelThreads * kSelPerWd;
+constexpr int kSelAgentsWe = kSelThreads * kSelPerWe;
+constexpr int kSelStageWd = kSelAgentsWd * 2;  // staged pairs per weekday selection block
+constexpr int kSelStageWe = kSelAgentsWe * 3;  // staged pairs per weekend selection block
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
 #pragma unroll
@@ -385,7 +402,8 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(DayState* __restrict__ st
                                                        const uint32_t* __restrict__ blk_counts,
                                                        const uint32_t* __restrict__ blk_ninf,
                                                        uint32_t* __restrict__ blk_off, int32_t* __restrict__ out_counts,
-                                                       int32_t* __restrict__ out_inf, int64_t N) {
+                                                       int32_t* __restrict__ out_inf, int64_t N, int32_t mark_limit_wd,
+                                                       int32_t mark_limit_we, int table_mode) {
     if (st->done) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr int NW = kPlanThreads / 64;
@@ -435,8 +453,18 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(DayState* __restrict__ st
         out_inf[day] = I;
         if (I == 0) {
             st->done = 1;
+            st->skip_full = 1;
+            st->skip_subset = 1;
+            st->skip_mark = 1;
         } else {
             const bool weekend = (day % 7) >= 5;
+            // Queried buckets are marked only when few enough infectors make
+            // a subset table likely to pay off; k_decide settles the path.
+            const bool mark = table_mode == TABLES_SUBSET ||
+                              (table_mode == TABLES_AUTO && I <= (weekend ? mark_limit_we : mark_limit_wd));
+            st->skip_mark = mark ? 0 : 1;
+            st->skip_full = mark ? 1 : 0;
+            st->skip_subset = 1;
             st->errand_off = st->rand_offset;
             st->rand_offset += weekend ? (uint64_t)(2 * N) : (uint64_t)(N / 2);
             st->contact_base = st->rand_offset;
@@ -460,7 +488,7 @@ __global__ void k_compact(const DayState* __restrict__ st, const int32_t* __rest
 // hour * E + (loc - base); hour 0 block [0, N), hour 1 block [N, 2N).
 __global__ __launch_bounds__(256) void k_gen_weekday(const Model* __restrict__ Mp, DayState* __restrict__ st,
                                                      const uint8_t* __restrict__ age, uint32_t* __restrict__ keys) {
-    if (st->done) return;
+    if (st->skip_full) return;
     const Model& M = *Mp;
     const int N = M.N;
     const uint64_t off = st->errand_off;
@@ -494,7 +522,7 @@ __global__ __launch_bounds__(256) void k_gen_weekday(const Model* __restrict__ M
 // counters errand_off + 2p, +1.  Keys: hour * E + (loc - base), 3 per person.
 __global__ __launch_bounds__(256) void k_gen_weekend(const Model* __restrict__ Mp, DayState* __restrict__ st,
                                                      uint32_t* __restrict__ keys) {
-    if (st->done) return;
+    if (st->skip_full) return;
     const Model& M = *Mp;
     const int N = M.N;
     const uint64_t off = st->errand_off;
@@ -515,6 +543,172 @@ __global__ __launch_bounds__(256) void k_gen_weekend(const Model* __restrict__ M
         }
     }
     if (bad) st->fault = 1;
+}
+
+// ---------------- subset errand tables ----------------
+
+__device__ __forceinline__ bool is_marked(const uint32_t* __restrict__ mark, uint32_t key) {
+    return (mark[key >> 5] >> (key & 31)) & 1u;
+}
+
+__global__ void k_clear_marks(const DayState* __restrict__ st, uint32_t* __restrict__ mark, int64_t nwords) {
+    if (st->done || st->skip_mark) return;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += stride) mark[i] = 0u;
+}
+
+// Chooses today's table path from the number of queried buckets.
+__global__ __launch_bounds__(kPlanThreads) void k_decide(DayState* __restrict__ st, const uint32_t* __restrict__ mark,
+                                                         int64_t nwords, int32_t K, int forced) {
+    if (st->done || st->skip_mark) return;
+    __shared__ uint32_t s_w[kPlanThreads / 64];
+    uint32_t c = 0;
+    for (int64_t i = threadIdx.x; i < nwords; i += kPlanThreads) c += (uint32_t)__popc(mark[i]);
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = 0;
+        for (int w = 0; w < kPlanThreads / 64; ++w) m += s_w[w];
+        st->n_marked = m;
+        const bool subset = forced == TABLES_SUBSET || (forced == TABLES_AUTO && (int64_t)m * 10 < (int64_t)K * 6);
+        st->skip_subset = subset ? 0 : 1;
+        st->skip_full = subset ? 1 : 0;
+    }
+}
+
+// Block-ordered selection of the pairs whose bucket is queried today.  A
+// block covers a contiguous agent range and stages its pairs by ascending
+// agent, so after compaction the ids ascend within every bucket (equal
+// buckets never come from two hours of one agent).
+template <int PER, bool WEEKEND>
+__device__ __forceinline__ void select_pairs(const Model& M, DayState* st, const uint8_t* __restrict__ age,
+                                             const uint32_t* __restrict__ mark, uint32_t* __restrict__ stage_keys,
+                                             int32_t* __restrict__ stage_ids, uint32_t* __restrict__ blk_cnt) {
+    constexpr int KPA = WEEKEND ? 3 : 2;  // keys per agent
+    constexpr int NK = PER * KPA;
+    const int N = M.N;
+    const uint64_t off = st->errand_off;
+    const int base = WEEKEND ? M.we_base : M.wd_base;
+    const int E = WEEKEND ? M.we_E : M.wd_E;
+    const int64_t a0 = (int64_t)blockIdx.x * (kSelThreads * PER) + (int64_t)threadIdx.x * PER;
+    uint32_t key[NK];
+    uint32_t sel = 0;
+    bool bad = false;
+    if (WEEKEND) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int64_t p = a0 + q;
+            key[3 * q] = key[3 * q + 1] = key[3 * q + 2] = 0;
+            if (p < N) {
+                int32_t hour[3], loc[3];
+                weekend_errands(M, off + (uint64_t)(int64_t)(2 * (int32_t)p), hour, loc);
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    int32_t l = loc[e] - base;
+                    if ((uint32_t)l >= (uint32_t)E) {
+                        bad = true;
+                        l = min(max(l, 0), E - 1);
+                    }
+                    const uint32_t k = (uint32_t)(hour[e] * E + l);
+                    key[3 * q + e] = k;
+                    if (is_marked(mark, k)) sel |= 1u << (3 * q + e);
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < PER / 2; ++q) {
+            const int64_t g = a0 / 2 + q;
+            key[4 * q] = key[4 * q + 1] = key[4 * q + 2] = key[4 * q + 3] = 0;
+            if (2 * g < N) {
+                const U4 u = draw4(off + (uint64_t)g, M.key0, M.key1);
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const int64_t p = 2 * g + k;
+                    if (p < N) {
+                        int32_t d1, d2;
+                        weekday_dests(M, u.w[2 * k], u.w[2 * k + 1], age[p] == kAgeAdult, d1, d2);
+                        int32_t l1 = d1 - base, l2 = d2 - base;
+                        if ((uint32_t)l1 >= (uint32_t)E || (uint32_t)l2 >= (uint32_t)E) {
+                            bad = true;
+                            l1 = min(max(l1, 0), E - 1);
+                            l2 = min(max(l2, 0), E - 1);
+                        }
+                        const int i = 4 * q + 2 * k;
+                        key[i] = (uint32_t)l1;
+                        key[i + 1] = (uint32_t)(E + l2);
+                        if (is_marked(mark, key[i])) sel |= 1u << i;
+                        if (is_marked(mark, key[i + 1])) sel |= 1u << (i + 1);
+                    }
+                }
+            }
+        }
+    }
+    // Block-wide exclusive scan of the per-thread counts.
+    __shared__ uint32_t s_w[kSelThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t cnt = (uint32_t)__popc(sel);
+    const uint32_t inc = wave_incl_scan(cnt, lane);
+    if (lane == 63) s_w[wave] = inc;
+    __syncthreads();
+    uint32_t pos = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kSelThreads / 64; ++w) {
+        if (w < wave) pos += s_w[w];
+        total += s_w[w];
+    }
+    pos += inc - cnt;
+    uint32_t* sk = stage_keys + (int64_t)blockIdx.x * (kSelThreads * NK);
+    int32_t* si = stage_ids + (int64_t)blockIdx.x * (kSelThreads * NK);
+#pragma unroll
+    for (int i = 0; i < NK; ++i)
+        if (sel & (1u << i)) {
+            sk[pos] = key[i];
+            si[pos] = (int32_t)(a0 + i / KPA);
+            pos++;
+        }
+    if (threadIdx.x == 0) blk_cnt[blockIdx.x] = total;
+    if (bad) st->fault = 1;
+}
+
+__global__ __launch_bounds__(kSelThreads) void k_select_weekday(const Model* __restrict__ Mp, DayState* __restrict__ st,
+                                                                const uint8_t* __restrict__ age,
+                                                                const uint32_t* __restrict__ mark,
+                                                                uint32_t* __restrict__ stage_keys,
+                                                                int32_t* __restrict__ stage_ids,
+                                                                uint32_t* __restrict__ blk_cnt) {
+    if (st->skip_subset) return;
+    select_pairs<kSelPerWd, false>(*Mp, st, age, mark, stage_keys, stage_ids, blk_cnt);
+}
+
+__global__ __launch_bounds__(kSelThreads) void k_select_weekend(const Model* __restrict__ Mp, DayState* __restrict__ st,
+                                                                const uint32_t* __restrict__ mark,
+                                                                uint32_t* __restrict__ stage_keys,
+                                                                int32_t* __restrict__ stage_ids,
+                                                                uint32_t* __restrict__ blk_cnt) {
+    if (st->skip_subset) return;
+    select_pairs<kSelPerWe, true>(*Mp, st, nullptr, mark, stage_keys, stage_ids, blk_cnt);
+}
+
+__global__ void k_set_nsel(DayState* __restrict__ st, const uint32_t* __restrict__ blk_cnt,
+                           const uint32_t* __restrict__ blk_off, int nblk) {
+    if (st->skip_subset) return;
+    st->n_sel = blk_off[nblk - 1] + blk_cnt[nblk - 1];
+}
+
+__global__ void k_compact_sel(const DayState* __restrict__ st, const uint32_t* __restrict__ stage_keys,
+                              const int32_t* __restrict__ stage_ids, const uint32_t* __restrict__ blk_cnt,
+                              const uint32_t* __restrict__ blk_off, uint32_t* __restrict__ sel_keys,
+                              int32_t* __restrict__ sel_ids, int stage_stride) {
+    if (st->skip_subset) return;
+    const uint32_t n = blk_cnt[blockIdx.x];
+    const uint32_t off = blk_off[blockIdx.x];
+    const int64_t src = (int64_t)blockIdx.x * stage_stride;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        sel_keys[off + i] = stage_keys[src + i];
+        sel_ids[off + i] = stage_ids[src + i];
+    }
 }
 
 // device_selectRandomPersonFromLocation (PandemicSim.cu:2028-2071) with
@@ -650,6 +844,72 @@ __device__ __forceinline__ bool can_infect(const Model& M, const Infectivity& f)
     const bool p = f.prof_p >= 0 && M.shed[f.prof_p][f.dd_p] * M.factor[0] > 0.f;
     const bool s = f.prof_s >= 0 && M.shed[f.prof_s][f.dd_s] * M.factor[1] > 0.f;
     return p || s;
+}
+
+__device__ __forceinline__ void set_mark(uint32_t* __restrict__ mark, int32_t key, int32_t K) {
+    if ((uint32_t)key >= (uint32_t)K) return;
+    const uint32_t bit = 1u << (key & 31);
+    if (!(__hip_atomic_load(&mark[key >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit))
+        atomicOr(&mark[key >> 5], bit);
+}
+
+// Marks the weekday errand buckets infectors draw contacts from: hour h is
+// used iff the errand profile asks for contacts in it (the workplace takes at
+// most 3 of the 5 non-household slots, so the first requested hour always
+// gets one; PandemicSim.cu:1776-1813).
+__global__ __launch_bounds__(256) void k_mark_weekday(const Model* __restrict__ Mp, const DayState* __restrict__ st,
+                                                      const int32_t* __restrict__ infected, AgentArrays A,
+                                                      uint32_t* __restrict__ mark) {
+    if (st->done || st->skip_mark) return;
+    const Model& M = *Mp;
+    const int I = st->n_infected;
+    const int day = st->day;
+    const uint64_t eoff = st->errand_off;
+    const int32_t K = kWdHours * M.wd_E;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < I; pos += stride) {
+        const int32_t idx = infected[pos];
+        const Infectivity f = infectivity(A, idx, day);
+        if (!can_infect(M, f)) continue;
+        const bool adult = A.age[idx] == kAgeAdult;
+        const U4 ue = draw4(eoff + (uint64_t)(idx / 2), M.key0, M.key1);
+        const int rs = 2 * (idx % 2);
+        int32_t d1, d2;
+        weekday_dests(M, ue.w[rs], ue.w[rs + 1], adult, d1, d2);
+        const int prof = adult ? (int)((ue.w[rs] ^ ue.w[rs + 1]) % 3u) : 3;
+        if (M.wd_assign[prof][0] > 0) set_mark(mark, d1 - M.wd_base, K);
+        if (M.wd_assign[prof][1] > 0) set_mark(mark, M.wd_E + d2 - M.wd_base, K);
+    }
+}
+
+// Marks the two weekend errand buckets of each infector (PandemicSim.cu:1891-1946).
+__global__ __launch_bounds__(256) void k_mark_weekend(const Model* __restrict__ Mp, const DayState* __restrict__ st,
+                                                      const int32_t* __restrict__ infected, AgentArrays A,
+                                                      uint32_t* __restrict__ mark) {
+    if (st->done || st->skip_mark) return;
+    const Model& M = *Mp;
+    const int I = st->n_infected;
+    const int day = st->day;
+    const uint64_t cbase = st->contact_base;
+    const uint64_t eoff = st->errand_off;
+    const int32_t K = kWeHours * M.we_E;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < I; pos += stride) {
+        const int32_t idx = infected[pos];
+        const Infectivity f = infectivity(A, idx, day);
+        if (!can_infect(M, f)) continue;
+        const U4 u = draw4(cbase + (uint64_t)(6 * pos), M.key0, M.key1);
+        const int prof = (int)(u.w[3] % 6u);
+        int32_t hour[3], loc[3];
+        weekend_errands(M, eoff + (uint64_t)(int64_t)(2 * idx), hour, loc);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int slot = M.we_assign[prof][k];
+            const int hsel = slot == 0 ? hour[0] : (slot == 1 ? hour[1] : hour[2]);
+            const int lsel = slot == 0 ? loc[0] : (slot == 1 ? loc[1] : loc[2]);
+            set_mark(mark, hsel * M.we_E + (lsel - M.we_base), K);
+        }
+    }
 }
 
 // D6w + D7: kernel_doWeekday (PandemicSim.cu:3236-3308) with
@@ -850,6 +1110,8 @@ __global__ void k_end_day(DayState* st) {
     st->day += 1;
     st->n_claimed[0] = 0;
     st->n_claimed[1] = 0;
+    st->skip_full = 1;
+    st->skip_subset = 1;
     if (st->day >= st->max_days) st->done = 1;
 }
 
@@ -922,7 +1184,7 @@ int round_half_up(double d) { return (int)floor(d + 0.5); }  // roundHalfUp_toIn
 const char* kKernelNames[KC_COUNT] = {"sweep",         "plan",       "compact",   "gen_weekday",
                                       "gen_weekend",   "sort_upsweep", "sort_scan", "sort_downsweep",
                                       "offsets",       "contacts_weekday", "contacts_weekend", "apply",
-                                      "end_day",       "setup"};
+                                      "end_day",       "mark",       "select",    "setup"};
 
 struct EventTimer : LaunchTimer {
     hipStream_t stream = nullptr;
@@ -1009,6 +1271,13 @@ struct pt_sim {
     uint64_t *claim_p = nullptr, *claim_s = nullptr;
     int32_t *list_p = nullptr, *list_s = nullptr;
     uint32_t *blk_counts = nullptr, *blk_ninf = nullptr, *blk_off = nullptr;
+    uint32_t* mark = nullptr;                                // queried-bucket bitmap
+    uint32_t* stage_keys = nullptr;                          // selection staging (per-block runs)
+    int32_t* stage_ids = nullptr;
+    uint32_t* sel_keys = nullptr;                            // 3N, compacted selection
+    int32_t* sel_ids = nullptr;                              // 3N
+    uint32_t *sblk_cnt = nullptr, *sblk_off = nullptr;       // per selection block
+    int table_mode = TABLES_AUTO;
     int nblk_sweep = 0;
     DayState* d_state = nullptr;
     int32_t *d_counts = nullptr, *d_inf = nullptr;  // [256*16], [256]
@@ -1041,6 +1310,10 @@ void dev_free(T*& p) {
     p = nullptr;
 }
 
+int grid_for(int64_t n, int threads, int cap) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>((n + threads - 1) / threads, cap));
+}
+
 // Buffers sized by the population (allocated in pt_sim_setup).
 void free_population(pt_sim* s) {
     dev_free(s->st_p);
@@ -1068,6 +1341,13 @@ void free_population(pt_sim* s) {
     dev_free(s->blk_counts);
     dev_free(s->blk_ninf);
     dev_free(s->blk_off);
+    dev_free(s->mark);
+    dev_free(s->stage_keys);
+    dev_free(s->stage_ids);
+    dev_free(s->sel_keys);
+    dev_free(s->sel_ids);
+    dev_free(s->sblk_cnt);
+    dev_free(s->sblk_off);
     table_scratch_free(s->tables);
     s->setup_done = s->seeded = s->have_snapshot = false;
 }
@@ -1111,32 +1391,63 @@ int enqueue_day(pt_sim* s, int d) {
     }
     {
         Scoped t(tm, KC_PLAN);
+        // Marking pays off while the expected queried fraction stays below
+        // about 0.4 (two queried buckets per infector): I <= K / 5.
+        const int32_t lim_wd = kWdHours * s->model.wd_E / 5, lim_we = kWeHours * s->model.we_E / 5;
         k_plan<<<1, kPlanThreads, 0, stc>>>(s->d_state, s->nblk_sweep, s->blk_counts, s->blk_ninf, s->blk_off,
-                                            s->d_counts, s->d_inf, s->N);
+                                            s->d_counts, s->d_inf, s->N, lim_wd, lim_we, s->table_mode);
     }
     {
         Scoped t(tm, KC_COMPACT);
         k_compact<<<s->nblk_sweep, 256, 0, stc>>>(s->d_state, s->staging, s->blk_ninf, s->blk_off, s->infected);
     }
-    int64_t n;
-    int32_t K;
-    IdMode mode;
+    const int64_t n = weekend ? 3 * s->N : 2 * s->N;
+    const int32_t K = weekend ? kWeHours * s->model.we_E : kWdHours * s->model.wd_E;
+    const IdMode mode = weekend ? ID_DIV3 : ID_HOUR_MAJOR;
+    AgentArrays A = agent_arrays(s, nullptr, nullptr);
+    DayState* st = s->d_state;
+    // Which buckets will be queried today, and which table path to take.
+    const int64_t nwords = ((int64_t)K + 31) / 32;
+    {
+        Scoped t(tm, KC_MARK);
+        k_clear_marks<<<grid_for(nwords, 256, 1024), 256, 0, stc>>>(st, s->mark, nwords);
+        if (weekend)
+            k_mark_weekend<<<s->grid_contacts, 256, 0, stc>>>(s->d_model, st, s->infected, A, s->mark);
+        else
+            k_mark_weekday<<<s->grid_contacts, 256, 0, stc>>>(s->d_model, st, s->infected, A, s->mark);
+        k_decide<<<1, kPlanThreads, 0, stc>>>(st, s->mark, nwords, K, s->table_mode);
+    }
+    // Full path: every agent's keys, implicit ids.
     if (weekend) {
         Scoped t(tm, KC_GEN_WEEKEND);
-        k_gen_weekend<<<s->grid_gen, 256, 0, stc>>>(s->d_model, s->d_state, s->keys);
-        n = 3 * s->N;
-        K = kWeHours * s->model.we_E;
-        mode = ID_DIV3;
+        k_gen_weekend<<<s->grid_gen, 256, 0, stc>>>(s->d_model, st, s->keys);
     } else {
         Scoped t(tm, KC_GEN_WEEKDAY);
-        k_gen_weekday<<<s->grid_gen, 256, 0, stc>>>(s->d_model, s->d_state, s->age, s->keys);
-        n = 2 * s->N;
-        K = kWdHours * s->model.wd_E;
-        mode = ID_HOUR_MAJOR;
+        k_gen_weekday<<<s->grid_gen, 256, 0, stc>>>(s->d_model, st, s->age, s->keys);
     }
-    hipError_t e = build_table(s->tables, s->keys, n, K, mode, N, s->er_people, s->er_off, &s->d_state->done, stc, tm);
+    hipError_t e = build_table(s->tables, s->keys, nullptr, n, nullptr, K, mode, N, s->er_people, s->er_off,
+                               &st->skip_full, stc, tm);
     if (e != hipSuccess) return fail(PT_EDEVICE, "table build: %s", hipGetErrorString(e));
-    AgentArrays A = agent_arrays(s, nullptr, nullptr);
+    // Subset path: only pairs of queried buckets, explicit ids.
+    {
+        Scoped t(tm, KC_SELECT);
+        const int nb = (int)((s->N + (weekend ? kSelAgentsWe : kSelAgentsWd) - 1) /
+                             (weekend ? kSelAgentsWe : kSelAgentsWd));
+        if (weekend)
+            k_select_weekend<<<nb, kSelThreads, 0, stc>>>(s->d_model, st, s->mark, s->stage_keys, s->stage_ids,
+                                                          s->sblk_cnt);
+        else
+            k_select_weekday<<<nb, kSelThreads, 0, stc>>>(s->d_model, st, s->age, s->mark, s->stage_keys, s->stage_ids,
+                                                          s->sblk_cnt);
+        e = exclusive_scan_u32(s->sblk_cnt, s->sblk_off, nb, s->tables.scan_part, &st->skip_subset, stc);
+        if (e != hipSuccess) return fail(PT_EDEVICE, "selection scan: %s", hipGetErrorString(e));
+        k_set_nsel<<<1, 1, 0, stc>>>(st, s->sblk_cnt, s->sblk_off, nb);
+        k_compact_sel<<<nb, 256, 0, stc>>>(st, s->stage_keys, s->stage_ids, s->sblk_cnt, s->sblk_off, s->sel_keys,
+                                           s->sel_ids, weekend ? kSelStageWe : kSelStageWd);
+    }
+    e = build_table(s->tables, s->sel_keys, s->sel_ids, n, &st->n_sel, K, mode, N, s->er_people, s->er_off,
+                    &st->skip_subset, stc, tm);
+    if (e != hipSuccess) return fail(PT_EDEVICE, "subset table build: %s", hipGetErrorString(e));
     if (weekend) {
         Scoped t(tm, KC_CONTACTS_WEEKEND);
         k_contacts_weekend<<<s->grid_contacts, kContactThreads, 0, stc>>>(s->d_model, s->d_state, s->infected, A);
@@ -1168,6 +1479,8 @@ int reset_day_state(pt_sim* s) {
     h.done = s->params.max_days <= 0 ? 1 : 0;
     h.rand_offset = s->rand_after_setup;
     h.max_days = s->params.max_days;
+    h.skip_full = 1;
+    h.skip_subset = 1;
     HIP_TRY(hipMemcpyAsync(s->d_state, &h, sizeof h, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->host_day = 0;
@@ -1179,10 +1492,6 @@ int read_state(pt_sim* s, DayState* h) {
     HIP_TRY(hipMemcpyAsync(h, s->d_state, sizeof *h, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return 0;
-}
-
-int grid_for(int64_t n, int threads, int cap) {
-    return (int)std::max<int64_t>(1, std::min<int64_t>((n + threads - 1) / threads, cap));
 }
 
 }  // namespace
@@ -1322,6 +1631,7 @@ int pt_sim_setup(pt_sim* s, int64_t* n_people_out) {
     const int64_t kmax = std::max<int64_t>((int64_t)kWdHours * m.wd_E, (int64_t)kWeHours * m.we_E);
     const int64_t n3 = 3 * N;
     const int64_t Npad = ((N + 15) / 16) * 16;  // the sweep reads whole uint4 words
+    const int64_t stage_n = std::max<int64_t>((N / kSelAgentsWd + 1) * kSelStageWd, (N / kSelAgentsWe + 1) * kSelStageWe);
     if (dev_alloc(&s->st_p, Npad) || dev_alloc(&s->st_s, Npad) || dev_alloc(&s->dy_p, Npad) ||
         dev_alloc(&s->dy_s, Npad) || dev_alloc(&s->gn_p, Npad) || dev_alloc(&s->gn_s, Npad) ||
         dev_alloc(&s->age, N) || dev_alloc(&s->snap, 6 * Npad) || dev_alloc(&s->household, N) ||
@@ -1331,7 +1641,11 @@ int pt_sim_setup(pt_sim* s, int64_t* n_people_out) {
         dev_alloc(&s->er_people, n3) || dev_alloc(&s->er_off, std::max<int64_t>(kmax, (int64_t)s->W) + 1) ||
         dev_alloc(&s->claim_p, N) || dev_alloc(&s->claim_s, N) || dev_alloc(&s->list_p, N) ||
         dev_alloc(&s->list_s, N) || dev_alloc(&s->blk_counts, (int64_t)s->nblk_sweep * 16) ||
-        dev_alloc(&s->blk_ninf, s->nblk_sweep) || dev_alloc(&s->blk_off, s->nblk_sweep)) {
+        dev_alloc(&s->blk_ninf, s->nblk_sweep) || dev_alloc(&s->blk_off, s->nblk_sweep) ||
+        dev_alloc(&s->mark, (kmax + 31) / 32 + 1) || dev_alloc(&s->stage_keys, stage_n) ||
+        dev_alloc(&s->stage_ids, stage_n) ||
+        dev_alloc(&s->sel_keys, n3) || dev_alloc(&s->sel_ids, n3) || dev_alloc(&s->sblk_cnt, N / kSelAgentsWe + 2) ||
+        dev_alloc(&s->sblk_off, N / kSelAgentsWe + 2)) {
         const std::string msg = g_err;
         free_population(s);
         return fail(PT_ENOMEM, "%s", msg.c_str());
@@ -1354,8 +1668,8 @@ int pt_sim_setup(pt_sim* s, int64_t* n_people_out) {
                                                                          s->keys);
     R += (uint64_t)(N / 4);
     s->rand_after_setup = R;
-    hipError_t e = build_table(s->tables, s->keys, N, s->W, ID_IDENTITY, (int32_t)N, s->wp_people, s->wp_off, nullptr,
-                               st, nullptr);
+    hipError_t e = build_table(s->tables, s->keys, nullptr, N, nullptr, s->W, ID_IDENTITY, (int32_t)N, s->wp_people,
+                               s->wp_off, nullptr, st, nullptr);
     if (e != hipSuccess) return fail(PT_EDEVICE, "workplace table: %s", hipGetErrorString(e));
     // K7: statuses; claim words start empty.
     k_init_status<<<grid_for(N, 256, 65535), 256, 0, st>>>(N, s->st_p, s->st_s, s->dy_p, s->dy_s, s->gn_p, s->gn_s);
@@ -1415,7 +1729,7 @@ int pt_sim_step(pt_sim* s, int32_t counts16_out[16], int32_t* infected_out) {
     HIP_TRY(hipMemcpy(&inf, s->d_inf + day, sizeof inf, hipMemcpyDeviceToHost));
     if (counts16_out) memcpy(counts16_out, c, sizeof c);
     if (infected_out) *infected_out = inf;
-    return h.done ? 1 : 0;
+    return inf == 0 ? 1 : 0;
 }
 
 int pt_sim_run(pt_sim* s, int32_t* counts_out, int32_t* infected_out, int32_t* days_out) {
@@ -1600,6 +1914,13 @@ int pt_sim_restore(pt_sim* s) {
     k_fill_u64<<<grid_for(N, 256, 65535), 256, 0, s->stream>>>(s->claim_s, N, ~0ull);
     HIP_TRY(hipGetLastError());
     return reset_day_state(s);
+}
+
+int pt_sim_set_table_mode(pt_sim* s, int32_t mode) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    if (mode < TABLES_AUTO || mode > TABLES_SUBSET) return fail(PT_EINVAL, "table mode %d unknown", mode);
+    s->table_mode = mode;
+    return PT_OK;
 }
 
 int pt_sim_enable_timing(pt_sim* s, int32_t enable) {

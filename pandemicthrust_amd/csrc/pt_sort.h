@@ -28,6 +28,8 @@ enum KernelClass : int {
     KC_CONTACTS_WEEKEND,
     KC_APPLY,
     KC_END_DAY,
+    KC_MARK,
+    KC_SELECT,
     KC_SETUP,
     KC_COUNT
 };
@@ -58,14 +60,19 @@ struct TableScratch {
 int table_scratch_alloc(TableScratch& s, int64_t n_cap);
 void table_scratch_free(TableScratch& s);
 
-// Builds the table.  keys is clobbered (it ends up holding the sorted keys).
+// Builds the table.  keys is clobbered.  vals (may be null) holds explicit
+// ids in input order; when null the ids follow from mode.  When n_dev is
+// set it holds the pair count on the device and n is only the capacity.
 // done (device int, may be null) turns every kernel into a no-op when set.
-hipError_t build_table(TableScratch& s, uint32_t* keys, int64_t n, int32_t K, IdMode mode, int32_t N,
-                       int32_t* people_out, int32_t* offsets_out, const int32_t* done, hipStream_t stream,
-                       LaunchTimer* tm);
+hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int64_t n, const uint32_t* n_dev,
+                       int32_t K, IdMode mode, int32_t N, int32_t* people_out, int32_t* offsets_out,
+                       const int32_t* done, hipStream_t stream, LaunchTimer* tm);
 
 // Number of radix passes build_table uses for K buckets.
 int radix_passes(int32_t K);
+
+// Widest digit (bits) of one radix pass, 4..11 (default 8).
+void set_radix_max_bits(int bits);
 
 // Exclusive scan of n uint32 values (in and out may alias); part needs
 // ceil(n / 8192) + 1 entries.

@@ -3,8 +3,8 @@
 //
 // Replaces cub::DeviceRadixSort::SortPairs + thrust::lower_bound
 // (PandemicSim.cu:1212-1248, :1316-1354, :3835-3862).  Keys are dense bucket
-// ids, so only ceil(log2 K) bits are sorted (2-3 passes of <= 8 bits instead
-// of the reference's 7 passes over all 32 bits).
+// ids, so only ceil(log2 K) bits are sorted (3 passes of <= 8 bits at 50M
+// agents instead of the reference's 7 passes over all 32 bits).
 #include "pt_sort.h"
 
 #include <algorithm>
@@ -17,8 +17,11 @@ constexpr int kThreads = 256;
 constexpr int kItems = 16;
 constexpr int kTile = kThreads * kItems;  // 4096 pairs per tile
 constexpr int kMaxSortBlocks = 1024;
-constexpr int kScanSeg = 8192;             // elements per scan block
+constexpr int kMaxBitsSupported = 11;
+constexpr int kScanSeg = 8192;  // elements per scan block
 constexpr int kScanItems = kScanSeg / kThreads;
+
+int g_max_bits = 8;  // widest digit per pass (see set_radix_max_bits)
 
 __device__ __forceinline__ int32_t implicit_id(int64_t j, int mode, int32_t N) {
     if (mode == ID_HOUR_MAJOR) return (int32_t)(j < N ? j : j - N);
@@ -27,6 +30,15 @@ __device__ __forceinline__ int32_t implicit_id(int64_t j, int mode, int32_t N) {
 }
 
 __device__ __forceinline__ bool is_done(const int32_t* done) { return done && *done; }
+
+// The block's slice of the n pairs (n from the device when n_dev is set).
+__device__ __forceinline__ void block_range(int64_t n_host, const uint32_t* n_dev, int64_t& begin, int64_t& end) {
+    const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
+    const int64_t tiles = (n + kTile - 1) / kTile;
+    const int64_t tpb = (tiles + gridDim.x - 1) / gridDim.x;
+    begin = (int64_t)blockIdx.x * tpb * kTile;
+    end = min(n, begin + tpb * kTile);
+}
 
 // Inclusive scan across a wave64.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
@@ -58,17 +70,17 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_w
 
 // Per-block digit histogram of the block's chunk.
 template <int BITS>
-__global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t* __restrict__ keys, int64_t n, int64_t chunk,
-                                                      int shift, uint32_t* __restrict__ hist, int nblk,
-                                                      const int32_t* done) {
+__global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t* __restrict__ keys, int64_t n_host,
+                                                      const uint32_t* __restrict__ n_dev, int shift,
+                                                      uint32_t* __restrict__ hist, int nblk, const int32_t* done) {
     if (is_done(done)) return;
     constexpr int RADIX = 1 << BITS;
     __shared__ uint32_t s_h[4][RADIX];
     const int tid = threadIdx.x, wave = tid >> 6;
     for (int i = tid; i < 4 * RADIX; i += kThreads) (&s_h[0][0])[i] = 0;
     __syncthreads();
-    const int64_t begin = (int64_t)blockIdx.x * chunk;
-    const int64_t end = min(n, begin + chunk);
+    int64_t begin, end;
+    block_range(n_host, n_dev, begin, end);
     const uint4* k4 = reinterpret_cast<const uint4*>(keys);
     for (int64_t q = begin / 4 + tid; q * 4 < end; q += kThreads) {
         if (q * 4 + 3 < end) {
@@ -87,17 +99,22 @@ __global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t* __restrict
 }
 
 // Stable scatter of the block's chunk by one digit.  Each tile of 4096 pairs
-// is ranked with wave64 ballots (match-any on the digit bits), staged in LDS
-// in digit order and written out as contiguous runs per digit.
+// is ranked per wave with ballots (match-any over the digit bits): the lowest
+// lane of each digit group adds the group's size to the wave's LDS counter
+// with a returning atomic, so the 16 rounds of a tile issue back to back and
+// LDS order gives each round the count of the rounds before it.  The tile is
+// then staged in LDS in digit order and written out as runs per digit, while
+// the next tile's pairs are already in flight.
 template <int BITS>
 __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restrict__ keys_in,
                                                         const int32_t* __restrict__ vals_in, int id_mode, int32_t N,
                                                         uint32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out,
-                                                        const uint32_t* __restrict__ hist, int64_t n, int64_t chunk,
-                                                        int shift, int nblk, const int32_t* done) {
+                                                        const uint32_t* __restrict__ hist, int64_t n_host,
+                                                        const uint32_t* __restrict__ n_dev, int shift, int nblk,
+                                                        const int32_t* done) {
     if (is_done(done)) return;
     constexpr int RADIX = 1 << BITS;
-    static_assert(RADIX <= kThreads, "one digit per thread in the digit scans");
+    constexpr int DPT = RADIX > kThreads ? RADIX / kThreads : 1;  // digits per thread in the digit scan
     constexpr uint32_t MASK = RADIX - 1;
     __shared__ uint32_t s_keys[kTile];
     __shared__ int32_t s_vals[kTile];
@@ -108,25 +125,41 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
     __shared__ uint32_t s_w[4];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t begin = (int64_t)blockIdx.x * chunk;
-    const int64_t end = min(n, begin + chunk);
-    if (tid < RADIX) s_base[tid] = hist[(int64_t)tid * nblk + blockIdx.x];
+    int64_t begin, end;
+    block_range(n_host, n_dev, begin, end);
+    for (int d = tid; d < RADIX; d += kThreads) s_base[d] = hist[(int64_t)d * nblk + blockIdx.x];
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+    uint32_t k[kItems];
+    int32_t v[kItems];
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+        const int64_t j = begin + wave * (kItems * 64) + r * 64 + lane;
+        const bool valid = j < end;
+        k[r] = valid ? keys_in[j] : 0u;
+        v[r] = vals_in ? (valid ? vals_in[j] : 0) : implicit_id(j, id_mode, N);
+    }
 
     for (int64_t t0 = begin; t0 < end; t0 += kTile) {
         const int tile_n = (int)min((int64_t)kTile, end - t0);
         for (int i = tid; i < 4 * RADIX; i += kThreads) (&s_wcnt[0][0])[i] = 0;
         __syncthreads();
-        uint32_t k[kItems];
-        int32_t v[kItems];
-        uint32_t rk[kItems];
+        // Next tile in flight while this one is ranked and written.
+        uint32_t nk[kItems];
+        int32_t nv[kItems];
+        const int64_t t1 = t0 + kTile;
+#pragma unroll
+        for (int r = 0; r < kItems; ++r) {
+            const int64_t j = t1 + wave * (kItems * 64) + r * 64 + lane;
+            const bool valid = j < end;
+            nk[r] = valid ? keys_in[j] : 0u;
+            nv[r] = vals_in ? (valid ? vals_in[j] : 0) : implicit_id(j, id_mode, N);
+        }
+        uint32_t old[kItems], info[kItems];
 #pragma unroll
         for (int r = 0; r < kItems; ++r) {
             const int e = wave * (kItems * 64) + r * 64 + lane;
             const bool valid = e < tile_n;
-            const int64_t j = t0 + e;
-            k[r] = valid ? keys_in[j] : 0u;
-            v[r] = vals_in ? (valid ? vals_in[j] : 0) : implicit_id(j, id_mode, N);
             const uint32_t d = (k[r] >> shift) & MASK;
             uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -135,30 +168,50 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
                 const uint64_t m = __ballot(bit);
                 peers &= bit ? m : ~m;
             }
-            const uint32_t old = s_wcnt[wave][d];
-            rk[r] = old + (uint32_t)__popcll(peers & below);
-            if (valid && (peers & below) == 0) s_wcnt[wave][d] = old + (uint32_t)__popcll(peers);
+            const uint32_t rk = (uint32_t)__popcll(peers & below);
+            const uint32_t leader = peers ? (uint32_t)(__ffsll((unsigned long long)peers) - 1) : 0u;
+            uint32_t o = 0;
+            if (valid && rk == 0) o = atomicAdd(&s_wcnt[wave][d], (uint32_t)__popcll(peers));
+            old[r] = o;
+            info[r] = rk | (leader << 8);
+        }
+        uint32_t rank[kItems];
+#pragma unroll
+        for (int r = 0; r < kItems; ++r) rank[r] = __shfl(old[r], (int)(info[r] >> 8), 64) + (info[r] & 0xFF);
+        __syncthreads();
+        for (int d = tid; d < RADIX; d += kThreads) {
+            const uint32_t c0 = s_wcnt[0][d], c1 = s_wcnt[1][d], c2 = s_wcnt[2][d], c3 = s_wcnt[3][d];
+            s_wcnt[0][d] = 0;
+            s_wcnt[1][d] = c0;
+            s_wcnt[2][d] = c0 + c1;
+            s_wcnt[3][d] = c0 + c1 + c2;
+            s_tot[d] = c0 + c1 + c2 + c3;
         }
         __syncthreads();
-        uint32_t tot = 0;
-        if (tid < RADIX) {
-            const uint32_t c0 = s_wcnt[0][tid], c1 = s_wcnt[1][tid], c2 = s_wcnt[2][tid], c3 = s_wcnt[3][tid];
-            s_wcnt[0][tid] = 0;
-            s_wcnt[1][tid] = c0;
-            s_wcnt[2][tid] = c0 + c1;
-            s_wcnt[3][tid] = c0 + c1 + c2;
-            tot = c0 + c1 + c2 + c3;
-            s_tot[tid] = tot;
+        {
+            uint32_t loc[DPT];
+            uint32_t sum = 0;
+#pragma unroll
+            for (int j = 0; j < DPT; ++j) {
+                const int d = tid * DPT + j;
+                loc[j] = d < RADIX ? s_tot[d] : 0u;
+                sum += loc[j];
+            }
+            uint32_t run = block_excl_scan256(sum, s_w, nullptr);
+#pragma unroll
+            for (int j = 0; j < DPT; ++j) {
+                const int d = tid * DPT + j;
+                if (d < RADIX) s_start[d] = run;
+                run += loc[j];
+            }
         }
-        const uint32_t st = block_excl_scan256(tot, s_w, nullptr);
-        if (tid < RADIX) s_start[tid] = st;
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < kItems; ++r) {
             const int e = wave * (kItems * 64) + r * 64 + lane;
             if (e < tile_n) {
                 const uint32_t d = (k[r] >> shift) & MASK;
-                const uint32_t pos = s_start[d] + s_wcnt[wave][d] + rk[r];
+                const uint32_t pos = s_start[d] + s_wcnt[wave][d] + rank[r];
                 s_keys[pos] = k[r];
                 s_vals[pos] = v[r];
             }
@@ -172,8 +225,12 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
             vals_out[g] = s_vals[i];
         }
         __syncthreads();
-        if (tid < RADIX) s_base[tid] += s_tot[tid];
-        __syncthreads();
+        for (int d = tid; d < RADIX; d += kThreads) s_base[d] += s_tot[d];
+#pragma unroll
+        for (int r = 0; r < kItems; ++r) {
+            k[r] = nk[r];
+            v[r] = nv[r];
+        }
     }
 }
 
@@ -233,9 +290,10 @@ __global__ __launch_bounds__(kThreads) void k_scan_down(const uint32_t* __restri
 }
 
 // offsets[q] = first position whose key is >= q, for q in (keys[i-1], keys[i]].
-__global__ void k_offsets(const uint32_t* __restrict__ keys, int64_t n, int32_t* __restrict__ offsets,
-                          const int32_t* done) {
+__global__ void k_offsets(const uint32_t* __restrict__ keys, int64_t n_host, const uint32_t* __restrict__ n_dev,
+                          int32_t* __restrict__ offsets, const int32_t* done) {
     if (is_done(done)) return;
+    const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const int64_t k = keys[i];
@@ -245,9 +303,10 @@ __global__ void k_offsets(const uint32_t* __restrict__ keys, int64_t n, int32_t*
 }
 
 // offsets[q] = n for q past the last key (and for all q when n == 0).
-__global__ void k_offsets_tail(const uint32_t* __restrict__ keys, int64_t n, int32_t K, int32_t* __restrict__ offsets,
-                               const int32_t* done) {
+__global__ void k_offsets_tail(const uint32_t* __restrict__ keys, int64_t n_host, const uint32_t* __restrict__ n_dev,
+                               int32_t K, int32_t* __restrict__ offsets, const int32_t* done) {
     if (is_done(done)) return;
+    const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
     const int64_t first = n == 0 ? 0 : (int64_t)keys[n - 1] + 1;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t q = first + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q <= K; q += stride)
@@ -267,11 +326,11 @@ struct Scoped {
 
 template <int BITS>
 hipError_t radix_pass(TableScratch& s, const uint32_t* kin, const int32_t* vin, int id_mode, int32_t N, uint32_t* kout,
-                      int32_t* vout, int64_t n, int shift, int nblk, int64_t chunk, const int32_t* done,
+                      int32_t* vout, int64_t n, const uint32_t* n_dev, int shift, int nblk, const int32_t* done,
                       hipStream_t st, LaunchTimer* tm) {
     {
         Scoped t(tm, KC_SORT_UPSWEEP);
-        k_upsweep<BITS><<<nblk, kThreads, 0, st>>>(kin, n, chunk, shift, s.hist, nblk, done);
+        k_upsweep<BITS><<<nblk, kThreads, 0, st>>>(kin, n, n_dev, shift, s.hist, nblk, done);
     }
     {
         Scoped t(tm, KC_SORT_SCAN);
@@ -280,39 +339,47 @@ hipError_t radix_pass(TableScratch& s, const uint32_t* kin, const int32_t* vin, 
     }
     {
         Scoped t(tm, KC_SORT_DOWNSWEEP);
-        k_downsweep<BITS><<<nblk, kThreads, 0, st>>>(kin, vin, id_mode, N, kout, vout, s.hist, n, chunk, shift, nblk,
+        k_downsweep<BITS><<<nblk, kThreads, 0, st>>>(kin, vin, id_mode, N, kout, vout, s.hist, n, n_dev, shift, nblk,
                                                      done);
     }
     return hipGetLastError();
 }
 
 hipError_t radix_pass_bits(int bits, TableScratch& s, const uint32_t* kin, const int32_t* vin, int id_mode, int32_t N,
-                           uint32_t* kout, int32_t* vout, int64_t n, int shift, int nblk, int64_t chunk,
+                           uint32_t* kout, int32_t* vout, int64_t n, const uint32_t* n_dev, int shift, int nblk,
                            const int32_t* done, hipStream_t st, LaunchTimer* tm) {
+#define PT_PASS(B) \
+    case B: return radix_pass<B>(s, kin, vin, id_mode, N, kout, vout, n, n_dev, shift, nblk, done, st, tm);
     switch (bits) {
-        case 1: return radix_pass<1>(s, kin, vin, id_mode, N, kout, vout, n, shift, nblk, chunk, done, st, tm);
-        case 2: return radix_pass<2>(s, kin, vin, id_mode, N, kout, vout, n, shift, nblk, chunk, done, st, tm);
-        case 3: return radix_pass<3>(s, kin, vin, id_mode, N, kout, vout, n, shift, nblk, chunk, done, st, tm);
-        case 4: return radix_pass<4>(s, kin, vin, id_mode, N, kout, vout, n, shift, nblk, chunk, done, st, tm);
-        case 5: return radix_pass<5>(s, kin, vin, id_mode, N, kout, vout, n, shift, nblk, chunk, done, st, tm);
-        case 6: return radix_pass<6>(s, kin, vin, id_mode, N, kout, vout, n, shift, nblk, chunk, done, st, tm);
-        case 7: return radix_pass<7>(s, kin, vin, id_mode, N, kout, vout, n, shift, nblk, chunk, done, st, tm);
-        case 8: return radix_pass<8>(s, kin, vin, id_mode, N, kout, vout, n, shift, nblk, chunk, done, st, tm);
+        PT_PASS(1) PT_PASS(2) PT_PASS(3) PT_PASS(4) PT_PASS(5) PT_PASS(6)
+        PT_PASS(7) PT_PASS(8) PT_PASS(9) PT_PASS(10) PT_PASS(11)
     }
+#undef PT_PASS
     return hipErrorInvalidValue;
+}
+
+int key_bits(int32_t K) {
+    int bits = 0;
+    while (((int64_t)1 << bits) < K) bits++;
+    return bits == 0 ? 1 : bits;
 }
 
 }  // namespace
 
+void set_radix_max_bits(int bits) { g_max_bits = std::min(std::max(bits, 4), kMaxBitsSupported); }
+
+int radix_passes(int32_t K) { return (key_bits(K) + g_max_bits - 1) / g_max_bits; }
+
 int table_scratch_alloc(TableScratch& s, int64_t n_cap) {
     table_scratch_free(s);
     const int64_t n = std::max<int64_t>(n_cap, 1);
+    const int64_t hist_n = ((int64_t)1 << kMaxBitsSupported) * kMaxSortBlocks;
     if (hipMalloc(&s.keys_b, n * sizeof(uint32_t)) != hipSuccess) return -1;
     if (hipMalloc(&s.vals_a, n * sizeof(int32_t)) != hipSuccess) return -1;
     if (hipMalloc(&s.vals_b, n * sizeof(int32_t)) != hipSuccess) return -1;
-    if (hipMalloc(&s.hist, (size_t)256 * kMaxSortBlocks * sizeof(uint32_t)) != hipSuccess) return -1;
-    // Partial sums for scans of up to max(n_cap, 256 * kMaxSortBlocks) values.
-    const int64_t parts = (std::max<int64_t>(n, (int64_t)256 * kMaxSortBlocks) + kScanSeg - 1) / kScanSeg + 2;
+    if (hipMalloc(&s.hist, (size_t)hist_n * sizeof(uint32_t)) != hipSuccess) return -1;
+    // Partial sums for scans of up to max(n_cap, hist_n) values.
+    const int64_t parts = (std::max<int64_t>(n, hist_n) + kScanSeg - 1) / kScanSeg + 2;
     if (hipMalloc(&s.scan_part, (size_t)parts * sizeof(uint32_t)) != hipSuccess) return -1;
     s.n_cap = n_cap;
     return 0;
@@ -337,35 +404,26 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint
     return hipGetLastError();
 }
 
-static int key_bits(int32_t K) {
-    int bits = 0;
-    while (((int64_t)1 << bits) < K) bits++;
-    return bits == 0 ? 1 : bits;
-}
-
-int radix_passes(int32_t K) { return (key_bits(K) + 7) / 8; }
-
-hipError_t build_table(TableScratch& s, uint32_t* keys, int64_t n, int32_t K, IdMode mode, int32_t N,
-                       int32_t* people_out, int32_t* offsets_out, const int32_t* done, hipStream_t stream,
-                       LaunchTimer* tm) {
+hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int64_t n, const uint32_t* n_dev,
+                       int32_t K, IdMode mode, int32_t N, int32_t* people_out, int32_t* offsets_out,
+                       const int32_t* done, hipStream_t stream, LaunchTimer* tm) {
     if (n > s.n_cap || K < 1 || n >= (int64_t)1 << 32) return hipErrorInvalidValue;
     const int bits = key_bits(K);
     const int passes = radix_passes(K);
     // Split the bits as evenly as possible over the passes.
-    int pass_bits[4] = {0, 0, 0, 0};
+    int pass_bits[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int p = 0; p < passes; ++p) pass_bits[p] = bits / passes + (p < bits % passes ? 1 : 0);
     uint32_t* kin = keys;
     if (n > 0) {
+        // n is the capacity when n_dev carries the actual count.
         const int64_t tiles = (n + kTile - 1) / kTile;
-        const int64_t tiles_per_blk = (tiles + kMaxSortBlocks - 1) / kMaxSortBlocks;
-        const int nblk = (int)((tiles + tiles_per_blk - 1) / tiles_per_blk);
-        const int64_t chunk = tiles_per_blk * kTile;
+        const int nblk = (int)std::min<int64_t>(tiles, kMaxSortBlocks);
         uint32_t* kout = s.keys_b;
-        const int32_t* vin = nullptr;
+        const int32_t* vin = vals;
         int shift = 0;
         for (int p = 0; p < passes; ++p) {
             int32_t* vout = (p == passes - 1) ? people_out : ((p & 1) ? s.vals_b : s.vals_a);
-            hipError_t e = radix_pass_bits(pass_bits[p], s, kin, vin, mode, N, kout, vout, n, shift, nblk, chunk, done,
+            hipError_t e = radix_pass_bits(pass_bits[p], s, kin, vin, mode, N, kout, vout, n, n_dev, shift, nblk, done,
                                            stream, tm);
             if (e != hipSuccess) return e;
             shift += pass_bits[p];
@@ -378,9 +436,9 @@ hipError_t build_table(TableScratch& s, uint32_t* keys, int64_t n, int32_t K, Id
         Scoped t(tm, KC_OFFSETS);
         if (n > 0) {
             const int blocks = (int)std::min<int64_t>((n + kThreads - 1) / kThreads, 4096);
-            k_offsets<<<blocks, kThreads, 0, stream>>>(kin, n, offsets_out, done);
+            k_offsets<<<blocks, kThreads, 0, stream>>>(kin, n, n_dev, offsets_out, done);
         }
-        k_offsets_tail<<<64, kThreads, 0, stream>>>(kin, n, K, offsets_out, done);
+        k_offsets_tail<<<64, kThreads, 0, stream>>>(kin, n, n_dev, K, offsets_out, done);
     }
     return hipGetLastError();
 }

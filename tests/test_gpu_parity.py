@@ -47,9 +47,10 @@ def test_setup_tables_bit_exact():
 
 
 def test_day_by_day_bit_exact():
-    """Each day: counts, infected list, the day's errand table and all state."""
+    """Each day: counts, infected list, the day's full errand table and all state."""
     from pandemicthrust_amd.api import PT_ERRAND_OFFSETS, PT_ERRAND_PEOPLE, PT_INFECTED
     gpu, ref = _sims(C0, 1)
+    gpu.set_table_mode(1)
     for day in range(14):  # two weeks: every weekday and weekend layout
         rg, cg, ig = gpu.step()
         rr, cr, ir = ref.step()
@@ -82,6 +83,22 @@ def test_full_run_bit_exact(row, seed):
     gp, gs = gpu.generation_offsets()
     rp, rs = ref.generation_offsets()
     assert np.array_equal(gp, rp) and np.array_equal(gs, rs)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_table_modes_give_identical_days(mode):
+    """Full tables, queried-bucket tables and the automatic choice produce
+    the same state every day (the tables differ only in unqueried buckets)."""
+    row = ["45", "2.2", "1.5", "20", "20", "0.1", "0.1", "0.8", "0.5"]
+    gpu, ref = _sims(row, 9)
+    gpu.set_table_mode(mode)
+    for day in range(45):
+        rg, cg, ig = gpu.step()
+        rr, cr, ir = ref.step()
+        assert (rg, ig) == (rr, ir) and np.array_equal(cg, cr), day
+        _assert_state(gpu, ref, "mode %d day %d" % (mode, day))
+        if rr == 1:
+            break
 
 
 def test_zero_reproduction_closed_form():
