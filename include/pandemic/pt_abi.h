@@ -127,10 +127,12 @@ int pt_sim_set_table_mode(pt_sim* sim, int32_t mode);
 int pt_sim_snapshot(pt_sim* sim);
 int pt_sim_restore(pt_sim* sim);
 
-/* Per-kernel-class timing with HIP events on the handle's stream.
- * enable != 0 starts recording (clears totals).  pt_sim_kernel_times
- * fills ms_out[k] (total milliseconds) and launches_out[k] for k < n and
- * returns the number of classes; pt_sim_kernel_name names class k. */
+/* Per-kernel-class timing with HIP events carried in the kernels' dispatch
+ * packets on the handle's stream (no extra barrier packets).  enable: 0 off,
+ * negative = every class, k + 1 = class k only; any call clears the totals.
+ * pt_sim_kernel_times fills ms_out[k] (total milliseconds) and
+ * launches_out[k] for k < n and returns the number of classes;
+ * pt_sim_kernel_name names class k. */
 int pt_sim_enable_timing(pt_sim* sim, int32_t enable);
 int pt_sim_kernel_times(pt_sim* sim, double* ms_out, int64_t* launches_out, int32_t n);
 const char* pt_sim_kernel_name(int32_t k);

@@ -195,8 +195,19 @@ class PandemicSim:
     def restore(self):
         _check(lib().pt_sim_restore(self.h))
 
-    def enable_timing(self, on=True):
-        _check(lib().pt_sim_enable_timing(self.h, 1 if on else 0))
+    def enable_timing(self, on=True, only=None):
+        """Times every kernel class, or only the class named `only`."""
+        if not on:
+            mode = 0
+        elif only is None:
+            mode = -1
+        else:
+            mode = self.kernel_classes().index(only) + 1
+        _check(lib().pt_sim_enable_timing(self.h, mode))
+
+    def kernel_classes(self):
+        n = _check(lib().pt_sim_kernel_times(self.h, None, None, 0))
+        return [lib().pt_sim_kernel_name(i).decode() for i in range(n)]
 
     def kernel_times(self):
         n = 32

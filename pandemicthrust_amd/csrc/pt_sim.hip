@@ -1186,13 +1186,12 @@ const char* kKernelNames[KC_COUNT] = {"sweep",         "plan",       "compact", 
                                       "offsets",       "contacts_weekday", "contacts_weekend", "apply",
                                       "end_day",       "mark",       "select",    "setup"};
 
+// Per-class kernel timing.  mode 0: off; -1: every class; k+1: class k only.
 struct EventTimer : LaunchTimer {
-    hipStream_t stream = nullptr;
-    bool on = false;
+    int mode = 0;
     std::vector<hipEvent_t> pool;
-    std::vector<std::pair<int, std::pair<size_t, size_t>>> pending;  // class, (begin event, end event)
+    std::vector<std::pair<int, size_t>> pending;  // class, index of the start event (stop = start + 1)
     size_t used = 0;
-    size_t open_begin = 0;
     double ms[KC_COUNT] = {0};
     int64_t launches[KC_COUNT] = {0};
     hipEvent_t next() {
@@ -1203,22 +1202,17 @@ struct EventTimer : LaunchTimer {
         }
         return pool[used++];
     }
-    void begin(int) override {
-        if (!on) return;
-        open_begin = used;
-        (void)hipEventRecord(next(), stream);
-    }
-    void end(int cls) override {
-        if (!on) return;
-        const size_t e = used;
-        (void)hipEventRecord(next(), stream);
-        pending.push_back({cls, {open_begin, e}});
+    bool want(int cls) const override { return mode == -1 || mode == cls + 1; }
+    void events(int cls, hipEvent_t* start, hipEvent_t* stop) override {
+        pending.push_back({cls, used});
+        *start = next();
+        *stop = next();
     }
     // Call after the stream is idle.
     void collect() {
         for (auto& p : pending) {
             float t = 0.f;
-            if (hipEventElapsedTime(&t, pool[p.second.first], pool[p.second.second]) == hipSuccess) {
+            if (hipEventElapsedTime(&t, pool[p.second], pool[p.second + 1]) == hipSuccess) {
                 ms[p.first] += t;
                 launches[p.first] += 1;
             }
@@ -1237,13 +1231,6 @@ struct EventTimer : LaunchTimer {
     ~EventTimer() {
         for (auto e : pool) (void)hipEventDestroy(e);
     }
-};
-
-struct Scoped {
-    LaunchTimer* tm;
-    int cls;
-    Scoped(LaunchTimer* t, int c) : tm(t), cls(c) { tm->begin(cls); }
-    ~Scoped() { tm->end(cls); }
 };
 
 }  // namespace
@@ -1384,87 +1371,70 @@ int enqueue_day(pt_sim* s, int d) {
     hipStream_t stc = s->stream;
     LaunchTimer* tm = &s->timer;
     const int N = (int)s->N;
-    {
-        Scoped t(tm, KC_SWEEP);
-        k_sweep<<<s->nblk_sweep, kSweepThreads, 0, stc>>>(s->N, s->d_state, s->st_p, s->st_s, s->dy_p, s->dy_s,
-                                                          s->blk_counts, s->blk_ninf, s->staging);
-    }
-    {
-        Scoped t(tm, KC_PLAN);
-        // Marking pays off while the expected queried fraction stays below
-        // about 0.4 (two queried buckets per infector): I <= K / 5.
-        const int32_t lim_wd = kWdHours * s->model.wd_E / 5, lim_we = kWeHours * s->model.we_E / 5;
-        k_plan<<<1, kPlanThreads, 0, stc>>>(s->d_state, s->nblk_sweep, s->blk_counts, s->blk_ninf, s->blk_off,
-                                            s->d_counts, s->d_inf, s->N, lim_wd, lim_we, s->table_mode);
-    }
-    {
-        Scoped t(tm, KC_COMPACT);
-        k_compact<<<s->nblk_sweep, 256, 0, stc>>>(s->d_state, s->staging, s->blk_ninf, s->blk_off, s->infected);
-    }
+    DayState* st = s->d_state;
+    launch(tm, KC_SWEEP, k_sweep, dim3(s->nblk_sweep), dim3(kSweepThreads), stc, s->N, st, s->st_p, s->st_s,
+           (const uint8_t*)s->dy_p, (const uint8_t*)s->dy_s, s->blk_counts, s->blk_ninf, s->staging);
+    // Marking pays off while the expected queried fraction stays below about
+    // 0.4 (two queried buckets per infector): I <= K / 5.
+    const int32_t lim_wd = kWdHours * s->model.wd_E / 5, lim_we = kWeHours * s->model.we_E / 5;
+    launch(tm, KC_PLAN, k_plan, dim3(1), dim3(kPlanThreads), stc, st, s->nblk_sweep, (const uint32_t*)s->blk_counts,
+           (const uint32_t*)s->blk_ninf, s->blk_off, s->d_counts, s->d_inf, s->N, lim_wd, lim_we, s->table_mode);
+    launch(tm, KC_COMPACT, k_compact, dim3(s->nblk_sweep), dim3(256), stc, (const DayState*)st,
+           (const int32_t*)s->staging, (const uint32_t*)s->blk_ninf, (const uint32_t*)s->blk_off, s->infected);
     const int64_t n = weekend ? 3 * s->N : 2 * s->N;
     const int32_t K = weekend ? kWeHours * s->model.we_E : kWdHours * s->model.wd_E;
     const IdMode mode = weekend ? ID_DIV3 : ID_HOUR_MAJOR;
     AgentArrays A = agent_arrays(s, nullptr, nullptr);
-    DayState* st = s->d_state;
+    const Model* M = s->d_model;
+    const int32_t* infected = s->infected;
     // Which buckets will be queried today, and which table path to take.
     const int64_t nwords = ((int64_t)K + 31) / 32;
-    {
-        Scoped t(tm, KC_MARK);
-        k_clear_marks<<<grid_for(nwords, 256, 1024), 256, 0, stc>>>(st, s->mark, nwords);
-        if (weekend)
-            k_mark_weekend<<<s->grid_contacts, 256, 0, stc>>>(s->d_model, st, s->infected, A, s->mark);
-        else
-            k_mark_weekday<<<s->grid_contacts, 256, 0, stc>>>(s->d_model, st, s->infected, A, s->mark);
-        k_decide<<<1, kPlanThreads, 0, stc>>>(st, s->mark, nwords, K, s->table_mode);
-    }
+    launch(tm, KC_MARK, k_clear_marks, dim3(grid_for(nwords, 256, 1024)), dim3(256), stc, (const DayState*)st, s->mark,
+           nwords);
+    if (weekend)
+        launch(tm, KC_MARK, k_mark_weekend, dim3(s->grid_contacts), dim3(256), stc, M, (const DayState*)st, infected, A,
+               s->mark);
+    else
+        launch(tm, KC_MARK, k_mark_weekday, dim3(s->grid_contacts), dim3(256), stc, M, (const DayState*)st, infected, A,
+               s->mark);
+    launch(tm, KC_MARK, k_decide, dim3(1), dim3(kPlanThreads), stc, st, (const uint32_t*)s->mark, nwords, K,
+           s->table_mode);
     // Full path: every agent's keys, implicit ids.
-    if (weekend) {
-        Scoped t(tm, KC_GEN_WEEKEND);
-        k_gen_weekend<<<s->grid_gen, 256, 0, stc>>>(s->d_model, st, s->keys);
-    } else {
-        Scoped t(tm, KC_GEN_WEEKDAY);
-        k_gen_weekday<<<s->grid_gen, 256, 0, stc>>>(s->d_model, st, s->age, s->keys);
-    }
+    if (weekend)
+        launch(tm, KC_GEN_WEEKEND, k_gen_weekend, dim3(s->grid_gen), dim3(256), stc, M, st, s->keys);
+    else
+        launch(tm, KC_GEN_WEEKDAY, k_gen_weekday, dim3(s->grid_gen), dim3(256), stc, M, st, (const uint8_t*)s->age,
+               s->keys);
     hipError_t e = build_table(s->tables, s->keys, nullptr, n, nullptr, K, mode, N, s->er_people, s->er_off,
                                &st->skip_full, stc, tm);
     if (e != hipSuccess) return fail(PT_EDEVICE, "table build: %s", hipGetErrorString(e));
     // Subset path: only pairs of queried buckets, explicit ids.
-    {
-        Scoped t(tm, KC_SELECT);
-        const int nb = (int)((s->N + (weekend ? kSelAgentsWe : kSelAgentsWd) - 1) /
-                             (weekend ? kSelAgentsWe : kSelAgentsWd));
-        if (weekend)
-            k_select_weekend<<<nb, kSelThreads, 0, stc>>>(s->d_model, st, s->mark, s->stage_keys, s->stage_ids,
-                                                          s->sblk_cnt);
-        else
-            k_select_weekday<<<nb, kSelThreads, 0, stc>>>(s->d_model, st, s->age, s->mark, s->stage_keys, s->stage_ids,
-                                                          s->sblk_cnt);
-        e = exclusive_scan_u32(s->sblk_cnt, s->sblk_off, nb, s->tables.scan_part, &st->skip_subset, stc);
-        if (e != hipSuccess) return fail(PT_EDEVICE, "selection scan: %s", hipGetErrorString(e));
-        k_set_nsel<<<1, 1, 0, stc>>>(st, s->sblk_cnt, s->sblk_off, nb);
-        k_compact_sel<<<nb, 256, 0, stc>>>(st, s->stage_keys, s->stage_ids, s->sblk_cnt, s->sblk_off, s->sel_keys,
-                                           s->sel_ids, weekend ? kSelStageWe : kSelStageWd);
-    }
+    const int nb = (int)((s->N + (weekend ? kSelAgentsWe : kSelAgentsWd) - 1) / (weekend ? kSelAgentsWe : kSelAgentsWd));
+    if (weekend)
+        launch(tm, KC_SELECT, k_select_weekend, dim3(nb), dim3(kSelThreads), stc, M, st, (const uint32_t*)s->mark,
+               s->stage_keys, s->stage_ids, s->sblk_cnt);
+    else
+        launch(tm, KC_SELECT, k_select_weekday, dim3(nb), dim3(kSelThreads), stc, M, st, (const uint8_t*)s->age,
+               (const uint32_t*)s->mark, s->stage_keys, s->stage_ids, s->sblk_cnt);
+    e = exclusive_scan_u32(s->sblk_cnt, s->sblk_off, nb, s->tables.scan_part, &st->skip_subset, stc, tm, KC_SELECT);
+    if (e != hipSuccess) return fail(PT_EDEVICE, "selection scan: %s", hipGetErrorString(e));
+    launch(tm, KC_SELECT, k_set_nsel, dim3(1), dim3(1), stc, st, (const uint32_t*)s->sblk_cnt,
+           (const uint32_t*)s->sblk_off, nb);
+    launch(tm, KC_SELECT, k_compact_sel, dim3(nb), dim3(256), stc, (const DayState*)st, (const uint32_t*)s->stage_keys,
+           (const int32_t*)s->stage_ids, (const uint32_t*)s->sblk_cnt, (const uint32_t*)s->sblk_off, s->sel_keys,
+           s->sel_ids, weekend ? kSelStageWe : kSelStageWd);
     e = build_table(s->tables, s->sel_keys, s->sel_ids, n, &st->n_sel, K, mode, N, s->er_people, s->er_off,
                     &st->skip_subset, stc, tm);
     if (e != hipSuccess) return fail(PT_EDEVICE, "subset table build: %s", hipGetErrorString(e));
-    if (weekend) {
-        Scoped t(tm, KC_CONTACTS_WEEKEND);
-        k_contacts_weekend<<<s->grid_contacts, kContactThreads, 0, stc>>>(s->d_model, s->d_state, s->infected, A);
-    } else {
-        Scoped t(tm, KC_CONTACTS_WEEKDAY);
-        k_contacts_weekday<<<s->grid_contacts, kContactThreads, 0, stc>>>(s->d_model, s->d_state, s->infected, A,
-                                                                           s->workplace_off);
-    }
-    {
-        Scoped t(tm, KC_APPLY);
-        k_apply<<<s->grid_contacts, 256, 0, stc>>>(s->d_state, s->claim_p, s->claim_s, s->list_p, s->list_s, s->st_p,
-                                                    s->st_s, s->dy_p, s->dy_s, s->gn_p, s->gn_s);
-    }
-    {
-        Scoped t(tm, KC_END_DAY);
-        k_end_day<<<1, 1, 0, stc>>>(s->d_state);
-    }
+    if (weekend)
+        launch(tm, KC_CONTACTS_WEEKEND, k_contacts_weekend, dim3(s->grid_contacts), dim3(kContactThreads), stc, M, st,
+               infected, A);
+    else
+        launch(tm, KC_CONTACTS_WEEKDAY, k_contacts_weekday, dim3(s->grid_contacts), dim3(kContactThreads), stc, M, st,
+               infected, A, s->workplace_off);
+    launch(tm, KC_APPLY, k_apply, dim3(s->grid_contacts), dim3(256), stc, (const DayState*)st, s->claim_p, s->claim_s,
+           (const int32_t*)s->list_p, (const int32_t*)s->list_s, s->st_p, s->st_s, s->dy_p, s->dy_s, s->gn_p, s->gn_s);
+    launch(tm, KC_END_DAY, k_end_day, dim3(1), dim3(1), stc, st);
     e = hipGetLastError();
     if (e != hipSuccess) return fail(PT_EDEVICE, "day launch: %s", hipGetErrorString(e));
     s->last_weekend = weekend ? 1 : 0;
@@ -1583,7 +1553,6 @@ int pt_sim_create(const pt_params* p, const int32_t seed_key[2], int32_t device,
         delete s;
         return fail(PT_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
     }
-    s->timer.stream = s->stream;
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0) {
         s->grid_contacts = cus * 8;
@@ -1926,7 +1895,7 @@ int pt_sim_set_table_mode(pt_sim* s, int32_t mode) {
 int pt_sim_enable_timing(pt_sim* s, int32_t enable) {
     if (!s) return fail(PT_EINVAL, "null handle");
     s->timer.reset();
-    s->timer.on = enable != 0;
+    s->timer.mode = enable < 0 ? -1 : enable;
     return PT_OK;
 }
 

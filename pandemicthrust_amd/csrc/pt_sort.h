@@ -8,6 +8,7 @@
 // offsets[K+1] with offsets[k] = number of keys < k.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -34,12 +35,26 @@ enum KernelClass : int {
     KC_COUNT
 };
 
-// Optional launch timer: begin/end bracket one launch on the stream.
+// Optional launch timer.  Timed launches carry their start/stop events in
+// the dispatch packet (hipExtLaunchKernel), so timing adds no barrier
+// packets between kernels.
 struct LaunchTimer {
-    virtual void begin(int cls) = 0;
-    virtual void end(int cls) = 0;
+    virtual bool want(int cls) const = 0;
+    // A fresh (start, stop) event pair accounted to class cls.
+    virtual void events(int cls, hipEvent_t* start, hipEvent_t* stop) = 0;
     virtual ~LaunchTimer() {}
 };
+
+template <typename F, typename... Args>
+inline void launch(LaunchTimer* tm, int cls, F kernel, dim3 grid, dim3 block, hipStream_t stream, Args... args) {
+    if (tm && tm->want(cls)) {
+        hipEvent_t a, b;
+        tm->events(cls, &a, &b);
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, stream, a, b, 0, args...);
+    } else {
+        kernel<<<grid, block, 0, stream>>>(args...);
+    }
+}
 
 // How a person id follows from the input position j.
 enum IdMode : int {
@@ -77,6 +92,6 @@ void set_radix_max_bits(int bits);
 // Exclusive scan of n uint32 values (in and out may alias); part needs
 // ceil(n / 8192) + 1 entries.
 hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* part, const int32_t* done,
-                              hipStream_t stream);
+                              hipStream_t stream, LaunchTimer* tm = nullptr, int cls = 0);
 
 }  // namespace pt

@@ -289,59 +289,36 @@ __global__ __launch_bounds__(kThreads) void k_scan_down(const uint32_t* __restri
     }
 }
 
-// offsets[q] = first position whose key is >= q, for q in (keys[i-1], keys[i]].
+// offsets[q] = first position whose key is >= q (thrust::lower_bound),
+// one binary search per bucket q in [0, K].
 __global__ void k_offsets(const uint32_t* __restrict__ keys, int64_t n_host, const uint32_t* __restrict__ n_dev,
-                          int32_t* __restrict__ offsets, const int32_t* done) {
+                          int32_t K, int32_t* __restrict__ offsets, const int32_t* done) {
     if (is_done(done)) return;
     const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int64_t k = keys[i];
-        const int64_t kp = i == 0 ? -1 : (int64_t)keys[i - 1];
-        for (int64_t q = kp + 1; q <= k; ++q) offsets[q] = (int32_t)i;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q <= K; q += stride) {
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if ((int64_t)keys[mid] < q)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        offsets[q] = (int32_t)lo;
     }
 }
-
-// offsets[q] = n for q past the last key (and for all q when n == 0).
-__global__ void k_offsets_tail(const uint32_t* __restrict__ keys, int64_t n_host, const uint32_t* __restrict__ n_dev,
-                               int32_t K, int32_t* __restrict__ offsets, const int32_t* done) {
-    if (is_done(done)) return;
-    const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
-    const int64_t first = n == 0 ? 0 : (int64_t)keys[n - 1] + 1;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t q = first + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q <= K; q += stride)
-        offsets[q] = (int32_t)n;
-}
-
-struct Scoped {
-    LaunchTimer* tm;
-    int cls;
-    Scoped(LaunchTimer* t, int c) : tm(t), cls(c) {
-        if (tm) tm->begin(cls);
-    }
-    ~Scoped() {
-        if (tm) tm->end(cls);
-    }
-};
 
 template <int BITS>
 hipError_t radix_pass(TableScratch& s, const uint32_t* kin, const int32_t* vin, int id_mode, int32_t N, uint32_t* kout,
                       int32_t* vout, int64_t n, const uint32_t* n_dev, int shift, int nblk, const int32_t* done,
                       hipStream_t st, LaunchTimer* tm) {
-    {
-        Scoped t(tm, KC_SORT_UPSWEEP);
-        k_upsweep<BITS><<<nblk, kThreads, 0, st>>>(kin, n, n_dev, shift, s.hist, nblk, done);
-    }
-    {
-        Scoped t(tm, KC_SORT_SCAN);
-        hipError_t e = exclusive_scan_u32(s.hist, s.hist, (int64_t)nblk << BITS, s.scan_part, done, st);
-        if (e != hipSuccess) return e;
-    }
-    {
-        Scoped t(tm, KC_SORT_DOWNSWEEP);
-        k_downsweep<BITS><<<nblk, kThreads, 0, st>>>(kin, vin, id_mode, N, kout, vout, s.hist, n, n_dev, shift, nblk,
-                                                     done);
-    }
+    launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS>, dim3(nblk), dim3(kThreads), st, kin, n, n_dev, shift, s.hist, nblk,
+           done);
+    hipError_t e = exclusive_scan_u32(s.hist, s.hist, (int64_t)nblk << BITS, s.scan_part, done, st, tm, KC_SORT_SCAN);
+    if (e != hipSuccess) return e;
+    launch(tm, KC_SORT_DOWNSWEEP, k_downsweep<BITS>, dim3(nblk), dim3(kThreads), st, kin, vin, id_mode, N, kout, vout,
+           (const uint32_t*)s.hist, n, n_dev, shift, nblk, done);
     return hipGetLastError();
 }
 
@@ -395,12 +372,12 @@ void table_scratch_free(TableScratch& s) {
 }
 
 hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* part, const int32_t* done,
-                              hipStream_t stream) {
+                              hipStream_t stream, LaunchTimer* tm, int cls) {
     if (n <= 0) return hipSuccess;
     const int nparts = (int)((n + kScanSeg - 1) / kScanSeg);
-    k_scan_reduce<<<nparts, kThreads, 0, stream>>>(in, n, part, done);
-    k_scan_parts<<<1, kThreads, 0, stream>>>(part, nparts, done);
-    k_scan_down<<<nparts, kThreads, 0, stream>>>(in, out, n, part, done);
+    launch(tm, cls, k_scan_reduce, dim3(nparts), dim3(kThreads), stream, in, n, part, done);
+    launch(tm, cls, k_scan_parts, dim3(1), dim3(kThreads), stream, part, nparts, done);
+    launch(tm, cls, k_scan_down, dim3(nparts), dim3(kThreads), stream, in, out, n, (const uint32_t*)part, done);
     return hipGetLastError();
 }
 
@@ -431,15 +408,10 @@ hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int
             std::swap(kin, kout);
         }
     }
-    {
-        // kin now holds the sorted keys (the caller's buffer or keys_b).
-        Scoped t(tm, KC_OFFSETS);
-        if (n > 0) {
-            const int blocks = (int)std::min<int64_t>((n + kThreads - 1) / kThreads, 4096);
-            k_offsets<<<blocks, kThreads, 0, stream>>>(kin, n, n_dev, offsets_out, done);
-        }
-        k_offsets_tail<<<64, kThreads, 0, stream>>>(kin, n, n_dev, K, offsets_out, done);
-    }
+    // kin now holds the sorted keys (the caller's buffer or keys_b).
+    const int oblocks = (int)std::min<int64_t>(((int64_t)K + kThreads) / kThreads, 4096);
+    launch(tm, KC_OFFSETS, k_offsets, dim3(oblocks), dim3(kThreads), stream, (const uint32_t*)kin, n, n_dev, K,
+           offsets_out, done);
     return hipGetLastError();
 }
 
