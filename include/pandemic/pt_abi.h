@@ -109,11 +109,15 @@ int pt_sim_copy_array(pt_sim* sim, int32_t which, void* dst, int64_t bytes);
  * [5] errand rand offset of the last day, [6] current day, [7] days run. */
 int pt_sim_get_scalars(pt_sim* sim, int64_t out[8]);
 
-/* Errand-table layout: [0] weekday base location, [1] weekday locations per
- * hour E_wd, [2] weekend base location, [3] weekend locations per hour E_we,
- * [4] weekday buckets (2 E_wd), [5] weekend buckets (10 E_we), [6] radix
- * passes of a weekday table, [7] radix passes of a weekend table.  Bucket
- * (hour, location) is stored densely at hour * E + (location - base). */
+/* Errand-table layout: [0] weekday base location, [1] weekday hour-0
+ * locations E_wd, [2] weekend base location, [3] weekend locations per hour
+ * E_we, [4] weekday buckets, [5] weekend buckets (10 E_we), [6] radix passes
+ * of a weekday table, [7] radix passes of a weekend table.  Weekend bucket
+ * (hour, location) is stored at hour * E_we + (location - base).  Weekday
+ * hour 0 is stored at location - base; weekday hour 1 stores only the
+ * locations adults can reach (children stay at their after-school site and
+ * make no hour-1 contacts), the rest share one last, never-queried bucket.
+ * The PT_ERRAND_* hooks always return the reference's layout. */
 int pt_sim_get_layout(pt_sim* sim, int64_t out[8]);
 
 /* Errand-table strategy: 0 automatic (default), 1 build every bucket each

@@ -49,8 +49,11 @@ struct Model {
     int32_t N, H, W;
     // Dense errand-table layout: every errand destination lies in
     // [base, W), so bucket (hour, loc) maps to hour * E + (loc - base).
+    // Weekday hour 1 keeps only [wd_base1, W) (weekday_keys).
     int32_t wd_base, wd_E;
     int32_t we_base, we_E;
+    int32_t wd_base1, wd_E1, wd_K;
+    int32_t pad1;
 };
 
 // ---------------- Threefry (Salmon et al., SC'11) ----------------
@@ -202,6 +205,23 @@ __device__ __forceinline__ void weekday_dests(const Model& M, uint32_t r1, uint3
                                               int32_t& d2) {
     d1 = fish_weekday_dest(M, r1, adult);
     d2 = adult ? fish_weekday_dest(M, r2, adult) : d1;
+}
+
+// Weekday errand-table keys of one person.  Hour 0 spans every destination
+// in [wd_base, W): key d1 - wd_base.  In hour 1 a child stays at its
+// after-school site and makes no contacts there (contact profile 3 has none
+// in hour 1, PandemicSim.cu:454-455), and adults only reach [wd_base1, W)
+// (wd_base1 = the lowest destination fish_weekday_dest can give an adult), so
+// no infector ever draws from an hour-1 bucket below wd_base1.  Hour 1 keeps
+// [wd_base1, W) at keys wd_E + (d2 - wd_base1); every hour-1 entry below it
+// goes to the last bucket wd_K - 1, which nobody queries.  Returns false for
+// a destination outside [wd_base, W) (the keys are then clamped in range).
+__device__ __forceinline__ bool weekday_keys(const Model& M, int32_t d1, int32_t d2, uint32_t& k0, uint32_t& k1) {
+    const int32_t l1 = d1 - M.wd_base, l2 = d2 - M.wd_base1;
+    const bool ok = (uint32_t)l1 < (uint32_t)M.wd_E && d2 >= M.wd_base && d2 < M.W;
+    k0 = (uint32_t)min(max(l1, 0), M.wd_E - 1);
+    k1 = l2 >= 0 ? (uint32_t)(M.wd_E + min(l2, M.wd_E1 - 1)) : (uint32_t)(M.wd_K - 1);
+    return ok;
 }
 
 // device_generateWeekendErrands (PandemicSim.cu:980-1014): three distinct

@@ -484,15 +484,14 @@ __global__ void k_compact(const DayState* __restrict__ st, const int32_t* __rest
 }
 
 // D3w: kernel_assignWeekdayAfterschoolAndErrands (PandemicSim.cu:2972-3022).
-// Person pair (2g, 2g+1) shares counter errand_off + g.  Keys are dense:
-// hour * E + (loc - base); hour 0 block [0, N), hour 1 block [N, 2N).
+// Person pair (2g, 2g+1) shares counter errand_off + g.  Keys from
+// weekday_keys; hour 0 block [0, N), hour 1 block [N, 2N).
 __global__ __launch_bounds__(256) void k_gen_weekday(const Model* __restrict__ Mp, DayState* __restrict__ st,
                                                      const uint8_t* __restrict__ age, uint32_t* __restrict__ keys) {
     if (st->skip_full) return;
     const Model& M = *Mp;
     const int N = M.N;
     const uint64_t off = st->errand_off;
-    const int base = M.wd_base, E = M.wd_E;
     bool bad = false;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; 2 * g < N; g += stride) {
@@ -504,14 +503,10 @@ __global__ __launch_bounds__(256) void k_gen_weekday(const Model* __restrict__ M
                 const bool adult = age[p] == kAgeAdult;
                 int32_t d1, d2;
                 weekday_dests(M, u.w[2 * k], u.w[2 * k + 1], adult, d1, d2);
-                int32_t l1 = d1 - base, l2 = d2 - base;
-                if ((uint32_t)l1 >= (uint32_t)E || (uint32_t)l2 >= (uint32_t)E) {
-                    bad = true;
-                    l1 = min(max(l1, 0), E - 1);
-                    l2 = min(max(l2, 0), E - 1);
-                }
-                keys[p] = (uint32_t)l1;
-                keys[N + p] = (uint32_t)(E + l2);
+                uint32_t k0, k1;
+                bad |= !weekday_keys(M, d1, d2, k0, k1);
+                keys[p] = k0;
+                keys[N + p] = k1;
             }
         }
     }
@@ -589,8 +584,7 @@ __device__ __forceinline__ void select_pairs(const Model& M, DayState* st, const
     constexpr int NK = PER * KPA;
     const int N = M.N;
     const uint64_t off = st->errand_off;
-    const int base = WEEKEND ? M.we_base : M.wd_base;
-    const int E = WEEKEND ? M.we_E : M.wd_E;
+    const int base = M.we_base, E = M.we_E;  // weekend layout
     const int64_t a0 = (int64_t)blockIdx.x * (kSelThreads * PER) + (int64_t)threadIdx.x * PER;
     uint32_t key[NK];
     uint32_t sel = 0;
@@ -629,15 +623,8 @@ __device__ __forceinline__ void select_pairs(const Model& M, DayState* st, const
                     if (p < N) {
                         int32_t d1, d2;
                         weekday_dests(M, u.w[2 * k], u.w[2 * k + 1], age[p] == kAgeAdult, d1, d2);
-                        int32_t l1 = d1 - base, l2 = d2 - base;
-                        if ((uint32_t)l1 >= (uint32_t)E || (uint32_t)l2 >= (uint32_t)E) {
-                            bad = true;
-                            l1 = min(max(l1, 0), E - 1);
-                            l2 = min(max(l2, 0), E - 1);
-                        }
                         const int i = 4 * q + 2 * k;
-                        key[i] = (uint32_t)l1;
-                        key[i + 1] = (uint32_t)(E + l2);
+                        bad |= !weekday_keys(M, d1, d2, key[i], key[i + 1]);
                         if (is_marked(mark, key[i])) sel |= 1u << i;
                         if (is_marked(mark, key[i + 1])) sel |= 1u << (i + 1);
                     }
@@ -865,7 +852,7 @@ __global__ __launch_bounds__(256) void k_mark_weekday(const Model* __restrict__ 
     const int I = st->n_infected;
     const int day = st->day;
     const uint64_t eoff = st->errand_off;
-    const int32_t K = kWdHours * M.wd_E;
+    const int32_t K = M.wd_K;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < I; pos += stride) {
         const int32_t idx = infected[pos];
@@ -876,9 +863,11 @@ __global__ __launch_bounds__(256) void k_mark_weekday(const Model* __restrict__ 
         const int rs = 2 * (idx % 2);
         int32_t d1, d2;
         weekday_dests(M, ue.w[rs], ue.w[rs + 1], adult, d1, d2);
+        uint32_t k0, k1;
+        weekday_keys(M, d1, d2, k0, k1);
         const int prof = adult ? (int)((ue.w[rs] ^ ue.w[rs + 1]) % 3u) : 3;
-        if (M.wd_assign[prof][0] > 0) set_mark(mark, d1 - M.wd_base, K);
-        if (M.wd_assign[prof][1] > 0) set_mark(mark, M.wd_E + d2 - M.wd_base, K);
+        if (M.wd_assign[prof][0] > 0) set_mark(mark, (int32_t)k0, K);
+        if (M.wd_assign[prof][1] > 0) set_mark(mark, (int32_t)k1, K);
     }
 }
 
@@ -965,7 +954,8 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts_weekday(const Mode
                 weekday_dests(M, ue.w[rs], ue.w[rs + 1], adult, d1, d2);
                 const int prof = adult ? (int)((ue.w[rs] ^ ue.w[rs + 1]) % 3u) : 3;
                 const int h0 = M.wd_assign[prof][0], h1 = M.wd_assign[prof][1];
-                const int32_t k0 = d1 - M.wd_base, k1 = M.wd_E + d2 - M.wd_base;
+                uint32_t k0, k1;
+                weekday_keys(M, d1, d2, k0, k1);
                 const int32_t e0off = A.er_off[k0], e0cnt = A.er_off[k0 + 1] - e0off;
                 const int32_t e1off = A.er_off[k1], e1cnt = A.er_off[k1 + 1] - e1off;
                 const int wtype = adult ? CT_WORKPLACE : CT_SCHOOL;
@@ -1254,7 +1244,7 @@ struct pt_sim {
     int32_t *infected = nullptr, *staging = nullptr;
     uint32_t* keys = nullptr;        // 3N
     int32_t* er_people = nullptr;    // 3N
-    int32_t* er_off = nullptr;       // dense, max(2 wd_E, 10 we_E) + 1
+    int32_t* er_off = nullptr;       // dense, max(wd_K, 10 we_E) + 1
     uint64_t *claim_p = nullptr, *claim_s = nullptr;
     int32_t *list_p = nullptr, *list_s = nullptr;
     uint32_t *blk_counts = nullptr, *blk_ninf = nullptr, *blk_off = nullptr;
@@ -1376,13 +1366,13 @@ int enqueue_day(pt_sim* s, int d) {
            (const uint8_t*)s->dy_p, (const uint8_t*)s->dy_s, s->blk_counts, s->blk_ninf, s->staging);
     // Marking pays off while the expected queried fraction stays below about
     // 0.4 (two queried buckets per infector): I <= K / 5.
-    const int32_t lim_wd = kWdHours * s->model.wd_E / 5, lim_we = kWeHours * s->model.we_E / 5;
+    const int32_t lim_wd = s->model.wd_K / 5, lim_we = kWeHours * s->model.we_E / 5;
     launch(tm, KC_PLAN, k_plan, dim3(1), dim3(kPlanThreads), stc, st, s->nblk_sweep, (const uint32_t*)s->blk_counts,
            (const uint32_t*)s->blk_ninf, s->blk_off, s->d_counts, s->d_inf, s->N, lim_wd, lim_we, s->table_mode);
     launch(tm, KC_COMPACT, k_compact, dim3(s->nblk_sweep), dim3(256), stc, (const DayState*)st,
            (const int32_t*)s->staging, (const uint32_t*)s->blk_ninf, (const uint32_t*)s->blk_off, s->infected);
     const int64_t n = weekend ? 3 * s->N : 2 * s->N;
-    const int32_t K = weekend ? kWeHours * s->model.we_E : kWdHours * s->model.wd_E;
+    const int32_t K = weekend ? kWeHours * s->model.we_E : s->model.wd_K;
     const IdMode mode = weekend ? ID_DIV3 : ID_HOUR_MAJOR;
     AgentArrays A = agent_arrays(s, nullptr, nullptr);
     const Model* M = s->d_model;
@@ -1462,6 +1452,48 @@ int read_state(pt_sim* s, DayState* h) {
     HIP_TRY(hipMemcpyAsync(h, s->d_state, sizeof *h, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return 0;
+}
+
+// Parity hook: the last weekday table in the reference's layout (bucket
+// hour * W + location, lower_bound offsets).  The reference's hour-1 bucket
+// of a location in [wd_base, wd_base1) holds exactly the people of its hour-0
+// bucket: only children reach those locations, and a child's hour-1
+// destination is its hour-0 one (weekday_dests).
+int expand_weekday_table(pt_sim* s, int32_t* offsets_out, int32_t* people_out, int64_t people_len) {
+    const Model& m = s->model;
+    const int W = s->W, E = m.wd_E, E1 = m.wd_E1, A = m.wd_base1 - m.wd_base;
+    std::vector<int32_t> dense((size_t)m.wd_K + 1);
+    HIP_TRY(hipMemcpy(dense.data(), s->er_off, dense.size() * 4, hipMemcpyDeviceToHost));
+    const int32_t h1 = dense[E];            // first hour-1 entry
+    const int32_t kids = dense[A] - dense[0];  // hour-0 entries below wd_base1
+    const int32_t end1 = dense[E + E1];     // end of the queried hour-1 buckets
+    if (offsets_out) {
+        for (int l = 0; l < W; ++l) {
+            offsets_out[l] = l < m.wd_base ? dense[0] : dense[l - m.wd_base];
+            int32_t v;
+            if (l < m.wd_base)
+                v = h1;
+            else if (l < m.wd_base1)
+                v = h1 + dense[l - m.wd_base] - dense[0];
+            else
+                v = dense[E + (l - m.wd_base1)] + kids;
+            offsets_out[W + l] = v;
+        }
+        offsets_out[2 * (int64_t)W] = end1 + kids;
+    }
+    if (people_out) {
+        const int64_t total = (int64_t)end1 + kids;
+        if (total > people_len) return fail(PT_EDEVICE, "weekday table holds %lld entries, %lld expected",
+                                            (long long)total, (long long)people_len);
+        std::vector<int32_t> dev((size_t)end1);
+        if (end1 > 0) HIP_TRY(hipMemcpy(dev.data(), s->er_people, (size_t)end1 * 4, hipMemcpyDeviceToHost));
+        int32_t* o = people_out;
+        o = std::copy(dev.begin(), dev.begin() + h1, o);
+        o = std::copy(dev.begin() + dense[0], dev.begin() + dense[A], o);
+        o = std::copy(dev.begin() + h1, dev.end(), o);
+        std::fill(o, people_out + people_len, 0);
+    }
+    return PT_OK;
 }
 
 }  // namespace
@@ -1548,6 +1580,14 @@ int pt_sim_create(const pt_params* p, const int32_t seed_key[2], int32_t device,
     m.wd_E = s->W - m.wd_base;
     m.we_base = m.wp_offset[kFirstErrandRow];
     m.we_E = s->W - m.we_base;
+    // Lowest weekday destination of an adult: the first errand location, or
+    // offset - 1 of an errand type without locations (fish_weekday_dest).
+    int32_t b1 = m.we_base;
+    for (int t = kFirstErrandRow; t < kTypes; ++t)
+        if (m.wp_count[t] == 0) b1 = std::min(b1, m.wp_offset[t] - 1);
+    m.wd_base1 = std::max(b1, m.wd_base);
+    m.wd_E1 = s->W - m.wd_base1;
+    m.wd_K = m.wd_E + m.wd_E1 + 1;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete s;
@@ -1597,7 +1637,7 @@ int pt_sim_setup(pt_sim* s, int64_t* n_people_out) {
     HIP_TRY(hipMemcpyAsync(s->d_model, &m, sizeof m, hipMemcpyHostToDevice, st));
     // Per-agent arrays (setup_sizeGlobalArrays, PandemicSim.cu:1047-1107).
     s->nblk_sweep = (int)((N + kSweepTile - 1) / kSweepTile);
-    const int64_t kmax = std::max<int64_t>((int64_t)kWdHours * m.wd_E, (int64_t)kWeHours * m.we_E);
+    const int64_t kmax = std::max<int64_t>((int64_t)m.wd_K, (int64_t)kWeHours * m.we_E);
     const int64_t n3 = 3 * N;
     const int64_t Npad = ((N + 15) / 16) * 16;  // the sweep reads whole uint4 words
     const int64_t stage_n = std::max<int64_t>((N / kSelAgentsWd + 1) * kSelStageWd, (N / kSelAgentsWe + 1) * kSelStageWe);
@@ -1802,22 +1842,25 @@ int pt_sim_copy_array(pt_sim* s, int32_t which, void* dst, int64_t bytes) {
         case PT_WP_OFFSETS: src = s->wp_off; break;
         case PT_WP_PEOPLE: src = s->wp_people; break;
         case PT_INFECTED: src = s->infected; break;
-        case PT_ERRAND_PEOPLE: src = s->er_people; break;
         case PT_HH_TYPES: src = s->hh_types; break;
+        case PT_ERRAND_PEOPLE:
+            if (s->last_weekend == 1) {
+                src = s->er_people;
+                break;
+            }
+            return expand_weekday_table(s, nullptr, (int32_t*)dst, need / 4);
         case PT_ERRAND_OFFSETS: {
+            if (s->last_weekend == 0) return expand_weekday_table(s, (int32_t*)dst, nullptr, 0);
             // Expand the dense bucket layout to the reference's
             // hour * W + location layout (lower_bound semantics).
-            const bool we = s->last_weekend == 1;
-            const int hours = we ? kWeHours : kWdHours;
-            const int E = we ? s->model.we_E : s->model.wd_E;
-            const int base = we ? s->model.we_base : s->model.wd_base;
-            std::vector<int32_t> dense((size_t)hours * E + 1);
+            const int E = s->model.we_E, base = s->model.we_base;
+            std::vector<int32_t> dense((size_t)kWeHours * E + 1);
             HIP_TRY(hipMemcpy(dense.data(), s->er_off, dense.size() * 4, hipMemcpyDeviceToHost));
             int32_t* out = (int32_t*)dst;
-            for (int hr = 0; hr < hours; ++hr)
+            for (int hr = 0; hr < kWeHours; ++hr)
                 for (int l = 0; l < s->W; ++l)
                     out[(int64_t)hr * s->W + l] = l < base ? dense[(size_t)hr * E] : dense[(size_t)hr * E + (l - base)];
-            out[(int64_t)hours * s->W] = dense[(size_t)hours * E];
+            out[(int64_t)kWeHours * s->W] = dense[(size_t)kWeHours * E];
             return PT_OK;
         }
         default: return fail(PT_EINVAL, "unknown array %d", which);
@@ -1848,7 +1891,7 @@ int pt_sim_get_layout(pt_sim* s, int64_t out[8]) {
     out[1] = m.wd_E;
     out[2] = m.we_base;
     out[3] = m.we_E;
-    out[4] = (int64_t)kWdHours * m.wd_E;
+    out[4] = m.wd_K;
     out[5] = (int64_t)kWeHours * m.we_E;
     out[6] = radix_passes((int32_t)out[4]);
     out[7] = radix_passes((int32_t)out[5]);
