@@ -33,12 +33,13 @@
 #include <vector>
 
 #include "../../include/pandemic/pt_abi.h"
+#include "pt_day.h"
 #include "pt_model.h"
 #include "pt_sort.h"
 
 using namespace pt;
 
-namespace {
+namespace pt {
 
 thread_local std::string g_err;
 
@@ -52,75 +53,10 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                              \
-    do {                                                                                           \
-        hipError_t e_ = (expr);                                                                    \
-        if (e_ != hipSuccess) return fail(PT_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
+}  // namespace pt
 
-constexpr int kSweepThreads = 512;
-constexpr int kSweepPer = 16;                             // agents per thread (one uint4 per array)
-constexpr int kSweepTile = kSweepThreads * kSweepPer;     // 8192 agents per block
-constexpr int kPlanThreads = 1024;
-constexpr int kContactThreads = 256;
+namespace {
 
-// The reference's host-side loop counters (PandemicSim.h:73-76,
-// simRandOffsetsStruct_t) kept in device memory.
-struct DayState {
-    int32_t day;           // current_day
-    int32_t done;          // the loop has ended
-    int32_t days_run;      // rows of output_infected_stats.csv produced
-    int32_t n_infected;    // infected_count of the current day
-    uint64_t rand_offset;  // rand_offset
-    uint64_t errand_off;   // errand_randOffset of the current day
-    uint64_t contact_base; // rand_offset at the contact kernel
-    uint32_t n_claimed[2]; // claim-list lengths (pandemic, seasonal)
-    int32_t max_days;
-    int32_t fault;         // nonzero if a kernel saw data outside the model's range
-    int32_t skip_full;     // the full errand-table path is idle today
-    int32_t skip_subset;   // the subset errand-table path is idle today
-    uint32_t n_sel;        // pairs selected into today's subset table
-    uint32_t n_marked;     // errand buckets queried today
-    int32_t skip_mark;     // too many infectors today for a subset table to pay off
-};
-
-// Errand tables are built either for every bucket (full) or only for the
-// buckets today's infectors draw contacts from (subset); the choice is
-// made on the device from the number of queried buckets.
-enum TableMode : int { TABLES_AUTO = 0, TABLES_FULL = 1, TABLES_SUBSET = 2 };
-constexpr int kSelThreads = 256;
-constexpr int kSelPerWd = 8;   // agents per thread, weekday selection
-constexpr int kSelPerWe = 4;   // agents per thread, weekend selection
-constexpr int kSelAgentsWd = kSelThreads * kSelPerWd;
-constexpr int kSelAgentsWe = kSelThreads * kSelPerWe;
-constexpr int kSelStageWd = kSelAgentsWd * 2;  // staged pairs per weekday selection block
-constexpr int kSelStageWe = kSelAgentsWe * 3;  // staged pairs per weekend selection block
-
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t t = __shfl_up(v, d, 64);
-        if (lane >= d) v += t;
-    }
-    return v;
-}
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
-
-// All 64 lanes must call: reserves n_mine slots per lane in a list whose
-// length lives at *counter; returns the lane's first slot.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, uint32_t n_mine, int lane) {
-    const uint32_t inc = wave_incl_scan(n_mine, lane);
-    const uint32_t total = __shfl(inc, 63, 64);
-    uint32_t base = 0;
-    if (lane == 0 && total) base = atomicAdd(counter, total);
-    base = __shfl(base, 0, 64);
-    return base + inc - n_mine;
-}
 
 // ============================ setup kernels ============================
 
@@ -227,40 +163,14 @@ __global__ void k_init_status(int64_t N, uint8_t* st_p, uint8_t* st_s, uint8_t* 
     }
 }
 
-// Plain copy of n16 16-byte words (snapshot / restore of the agent state).
-__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
-}
-
 __global__ void k_fill_u64(uint64_t* p, int64_t n, uint64_t v) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
-// setup_initialInfected (PandemicSim.cu:708-758).
-__global__ void k_seed_infections(const int32_t* __restrict__ ids, int n_p, int n_s, uint8_t* st_p, uint8_t* st_s,
-                                  uint8_t* dy_p, uint8_t* dy_s, uint8_t* gn_p, uint8_t* gn_s) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_p + n_s) return;
-    const int id = ids[i];
-    if (i < n_p) {
-        st_p[id] = kInfected;
-        dy_p[id] = 0;
-        gn_p[id] = 0;
-    } else {
-        st_s[id] = kInfected;
-        dy_s[id] = 0;
-        gn_s[id] = 0;
-    }
-}
-
 // ============================ daily kernels ============================
 
-// D1 + D2 fused: recover (day <= current_day - 10), count the 8 status codes
-// per strain and stage the ids of infected agents per block.
-// (kernel_countInfectedStatusAndRecover PandemicSim.cu:2097-2192,
-//  daily_buildInfectedArray_global :2207-2224.)
+// D1 + D2 fused over the whole population (sweep_block).
 __global__ __launch_bounds__(kSweepThreads) void k_sweep(int64_t N, DayState* __restrict__ st,
                                                          uint8_t* __restrict__ st_p, uint8_t* __restrict__ st_s,
                                                          const uint8_t* __restrict__ dy_p,
@@ -269,129 +179,9 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(int64_t N, DayState* __
                                                          uint32_t* __restrict__ blk_ninf,
                                                          int32_t* __restrict__ staging) {
     if (st->done) return;
-    const int dtr = st->day - kCulmination;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t a0 = (int64_t)blockIdx.x * kSweepTile + (int64_t)tid * kSweepPer;
-    uint8_t sp[kSweepPer], ss[kSweepPer];
-    const bool full = a0 + kSweepPer <= N;
-    if (full) {
-        const uint4 vp = *reinterpret_cast<const uint4*>(st_p + a0);
-        const uint4 vs = *reinterpret_cast<const uint4*>(st_s + a0);
-        memcpy(sp, &vp, 16);
-        memcpy(ss, &vs, 16);
-    } else {
-#pragma unroll
-        for (int i = 0; i < kSweepPer; ++i) {
-            sp[i] = (a0 + i < N) ? st_p[a0 + i] : kSusceptible;
-            ss[i] = (a0 + i < N) ? st_s[a0 + i] : kSusceptible;
-        }
-    }
-    if (dtr >= 0) {
-        bool anyp = false, anys = false;
-#pragma unroll
-        for (int i = 0; i < kSweepPer; ++i) {
-            anyp |= sp[i] >= kInfected;
-            anys |= ss[i] >= kInfected;
-        }
-        if (anyp) {
-            uint8_t dp[kSweepPer];
-            if (full) {
-                const uint4 v = *reinterpret_cast<const uint4*>(dy_p + a0);
-                memcpy(dp, &v, 16);
-            } else {
-#pragma unroll
-                for (int i = 0; i < kSweepPer; ++i) dp[i] = (a0 + i < N) ? dy_p[a0 + i] : kDayNone;
-            }
-            bool dirty = false;
-#pragma unroll
-            for (int i = 0; i < kSweepPer; ++i)
-                if (sp[i] >= kInfected && (int)dp[i] <= dtr) {
-                    sp[i] = kRecovered;
-                    dirty = true;
-                }
-            if (dirty) {
-                if (full) {
-                    uint4 v;
-                    memcpy(&v, sp, 16);
-                    *reinterpret_cast<uint4*>(st_p + a0) = v;
-                } else {
-                    for (int i = 0; i < kSweepPer; ++i)
-                        if (a0 + i < N) st_p[a0 + i] = sp[i];
-                }
-            }
-        }
-        if (anys) {
-            uint8_t ds[kSweepPer];
-            if (full) {
-                const uint4 v = *reinterpret_cast<const uint4*>(dy_s + a0);
-                memcpy(ds, &v, 16);
-            } else {
-#pragma unroll
-                for (int i = 0; i < kSweepPer; ++i) ds[i] = (a0 + i < N) ? dy_s[a0 + i] : kDayNone;
-            }
-            bool dirty = false;
-#pragma unroll
-            for (int i = 0; i < kSweepPer; ++i)
-                if (ss[i] >= kInfected && (int)ds[i] <= dtr) {
-                    ss[i] = kRecovered;
-                    dirty = true;
-                }
-            if (dirty) {
-                if (full) {
-                    uint4 v;
-                    memcpy(&v, ss, 16);
-                    *reinterpret_cast<uint4*>(st_s + a0) = v;
-                } else {
-                    for (int i = 0; i < kSweepPer; ++i)
-                        if (a0 + i < N) st_s[a0 + i] = ss[i];
-                }
-            }
-        }
-    }
-    // Counts: 8 bins per strain packed as bytes (at most 16 per thread).
-    uint64_t cp = 0, cs = 0;
-    uint32_t inf_mask = 0;
-#pragma unroll
-    for (int i = 0; i < kSweepPer; ++i) {
-        if (a0 + i < N) {
-            cp += 1ull << (8 * (sp[i] & 7));
-            cs += 1ull << (8 * (ss[i] & 7));
-            if (sp[i] >= kInfected || ss[i] >= kInfected) inf_mask |= 1u << i;
-        }
-    }
-    __shared__ uint32_t s_cnt[kSweepThreads / 64][16];
-    __shared__ uint32_t s_inf[kSweepThreads / 64];
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-        const uint32_t vp = wave_sum((uint32_t)((cp >> (8 * b)) & 0xFF));
-        const uint32_t vs = wave_sum((uint32_t)((cs >> (8 * b)) & 0xFF));
-        if (lane == 0) {
-            s_cnt[wave][b] = vp;
-            s_cnt[wave][8 + b] = vs;
-        }
-    }
-    const uint32_t ninf = (uint32_t)__popc(inf_mask);
-    const uint32_t inc = wave_incl_scan(ninf, lane);
-    if (lane == 63) s_inf[wave] = inc;
-    __syncthreads();
-    uint32_t wbase = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < kSweepThreads / 64; ++w) {
-        if (w < wave) wbase += s_inf[w];
-        total += s_inf[w];
-    }
-    uint32_t pos = wbase + inc - ninf;
-    int32_t* out = staging + (int64_t)blockIdx.x * kSweepTile;
-#pragma unroll
-    for (int i = 0; i < kSweepPer; ++i)
-        if (inf_mask & (1u << i)) out[pos++] = (int32_t)(a0 + i);
-    if (tid < 16) {
-        uint32_t s = 0;
-#pragma unroll
-        for (int w = 0; w < kSweepThreads / 64; ++w) s += s_cnt[w][tid];
-        blk_counts[(int64_t)blockIdx.x * 16 + tid] = s;
-    }
-    if (tid == 0) blk_ninf[blockIdx.x] = total;
+    const int64_t b = blockIdx.x;
+    sweep_block(N, st->day, b * kSweepTile, st_p, st_s, dy_p, dy_s, blk_counts + b * 16, blk_ninf + b,
+                staging + b * kSweepTile, 0);
 }
 
 // Day plan (one block): daily counts, the infected count and block offsets
@@ -542,10 +332,6 @@ __global__ __launch_bounds__(256) void k_gen_weekend(const Model* __restrict__ M
 
 // ---------------- subset errand tables ----------------
 
-__device__ __forceinline__ bool is_marked(const uint32_t* __restrict__ mark, uint32_t key) {
-    return (mark[key >> 5] >> (key & 31)) & 1u;
-}
-
 __global__ void k_clear_marks(const DayState* __restrict__ st, uint32_t* __restrict__ mark, int64_t nwords) {
     if (st->done || st->skip_mark) return;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -572,93 +358,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_decide(DayState* __restrict__ 
     }
 }
 
-// Block-ordered selection of the pairs whose bucket is queried today.  A
-// block covers a contiguous agent range and stages its pairs by ascending
-// agent, so after compaction the ids ascend within every bucket (equal
-// buckets never come from two hours of one agent).
-template <int PER, bool WEEKEND>
-__device__ __forceinline__ void select_pairs(const Model& M, DayState* st, const uint8_t* __restrict__ age,
-                                             const uint32_t* __restrict__ mark, uint32_t* __restrict__ stage_keys,
-                                             int32_t* __restrict__ stage_ids, uint32_t* __restrict__ blk_cnt) {
-    constexpr int KPA = WEEKEND ? 3 : 2;  // keys per agent
-    constexpr int NK = PER * KPA;
-    const int N = M.N;
-    const uint64_t off = st->errand_off;
-    const int base = M.we_base, E = M.we_E;  // weekend layout
-    const int64_t a0 = (int64_t)blockIdx.x * (kSelThreads * PER) + (int64_t)threadIdx.x * PER;
-    uint32_t key[NK];
-    uint32_t sel = 0;
-    bool bad = false;
-    if (WEEKEND) {
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int64_t p = a0 + q;
-            key[3 * q] = key[3 * q + 1] = key[3 * q + 2] = 0;
-            if (p < N) {
-                int32_t hour[3], loc[3];
-                weekend_errands(M, off + (uint64_t)(int64_t)(2 * (int32_t)p), hour, loc);
-#pragma unroll
-                for (int e = 0; e < 3; ++e) {
-                    int32_t l = loc[e] - base;
-                    if ((uint32_t)l >= (uint32_t)E) {
-                        bad = true;
-                        l = min(max(l, 0), E - 1);
-                    }
-                    const uint32_t k = (uint32_t)(hour[e] * E + l);
-                    key[3 * q + e] = k;
-                    if (is_marked(mark, k)) sel |= 1u << (3 * q + e);
-                }
-            }
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < PER / 2; ++q) {
-            const int64_t g = a0 / 2 + q;
-            key[4 * q] = key[4 * q + 1] = key[4 * q + 2] = key[4 * q + 3] = 0;
-            if (2 * g < N) {
-                const U4 u = draw4(off + (uint64_t)g, M.key0, M.key1);
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    const int64_t p = 2 * g + k;
-                    if (p < N) {
-                        int32_t d1, d2;
-                        weekday_dests(M, u.w[2 * k], u.w[2 * k + 1], age[p] == kAgeAdult, d1, d2);
-                        const int i = 4 * q + 2 * k;
-                        bad |= !weekday_keys(M, d1, d2, key[i], key[i + 1]);
-                        if (is_marked(mark, key[i])) sel |= 1u << i;
-                        if (is_marked(mark, key[i + 1])) sel |= 1u << (i + 1);
-                    }
-                }
-            }
-        }
-    }
-    // Block-wide exclusive scan of the per-thread counts.
-    __shared__ uint32_t s_w[kSelThreads / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t cnt = (uint32_t)__popc(sel);
-    const uint32_t inc = wave_incl_scan(cnt, lane);
-    if (lane == 63) s_w[wave] = inc;
-    __syncthreads();
-    uint32_t pos = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < kSelThreads / 64; ++w) {
-        if (w < wave) pos += s_w[w];
-        total += s_w[w];
-    }
-    pos += inc - cnt;
-    uint32_t* sk = stage_keys + (int64_t)blockIdx.x * (kSelThreads * NK);
-    int32_t* si = stage_ids + (int64_t)blockIdx.x * (kSelThreads * NK);
-#pragma unroll
-    for (int i = 0; i < NK; ++i)
-        if (sel & (1u << i)) {
-            sk[pos] = key[i];
-            si[pos] = (int32_t)(a0 + i / KPA);
-            pos++;
-        }
-    if (threadIdx.x == 0) blk_cnt[blockIdx.x] = total;
-    if (bad) st->fault = 1;
-}
-
+// Subset path: stage the pairs of the queried buckets (select_block).
 __global__ __launch_bounds__(kSelThreads) void k_select_weekday(const Model* __restrict__ Mp, DayState* __restrict__ st,
                                                                 const uint8_t* __restrict__ age,
                                                                 const uint32_t* __restrict__ mark,
@@ -666,7 +366,11 @@ __global__ __launch_bounds__(kSelThreads) void k_select_weekday(const Model* __r
                                                                 int32_t* __restrict__ stage_ids,
                                                                 uint32_t* __restrict__ blk_cnt) {
     if (st->skip_subset) return;
-    select_pairs<kSelPerWd, false>(*Mp, st, age, mark, stage_keys, stage_ids, blk_cnt);
+    const Model& M = *Mp;
+    const int64_t b = blockIdx.x;
+    if (!select_block<kSelPerWd, false>(M, st->errand_off, age, mark, 0u, false, b * kSelAgentsWd, M.N,
+                                        stage_keys + b * kSelStageWd, stage_ids + b * kSelStageWd, blk_cnt + b))
+        st->fault = 1;
 }
 
 __global__ __launch_bounds__(kSelThreads) void k_select_weekend(const Model* __restrict__ Mp, DayState* __restrict__ st,
@@ -675,7 +379,11 @@ __global__ __launch_bounds__(kSelThreads) void k_select_weekend(const Model* __r
                                                                 int32_t* __restrict__ stage_ids,
                                                                 uint32_t* __restrict__ blk_cnt) {
     if (st->skip_subset) return;
-    select_pairs<kSelPerWe, true>(*Mp, st, nullptr, mark, stage_keys, stage_ids, blk_cnt);
+    const Model& M = *Mp;
+    const int64_t b = blockIdx.x;
+    if (!select_block<kSelPerWe, true>(M, st->errand_off, nullptr, mark, 0u, false, b * kSelAgentsWe, M.N,
+                                       stage_keys + b * kSelStageWe, stage_ids + b * kSelStageWe, blk_cnt + b))
+        st->fault = 1;
 }
 
 __global__ void k_set_nsel(DayState* __restrict__ st, const uint32_t* __restrict__ blk_cnt,
@@ -684,166 +392,7 @@ __global__ void k_set_nsel(DayState* __restrict__ st, const uint32_t* __restrict
     st->n_sel = blk_off[nblk - 1] + blk_cnt[nblk - 1];
 }
 
-__global__ void k_compact_sel(const DayState* __restrict__ st, const uint32_t* __restrict__ stage_keys,
-                              const int32_t* __restrict__ stage_ids, const uint32_t* __restrict__ blk_cnt,
-                              const uint32_t* __restrict__ blk_off, uint32_t* __restrict__ sel_keys,
-                              int32_t* __restrict__ sel_ids, int stage_stride) {
-    if (st->skip_subset) return;
-    const uint32_t n = blk_cnt[blockIdx.x];
-    const uint32_t off = blk_off[blockIdx.x];
-    const int64_t src = (int64_t)blockIdx.x * stage_stride;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        sel_keys[off + i] = stage_keys[src + i];
-        sel_ids[off + i] = stage_ids[src + i];
-    }
-}
-
-// device_selectRandomPersonFromLocation (PandemicSim.cu:2028-2071) with
-// device_getVictimAtIndex (:2016-2025): household members are consecutive ids.
-__device__ __forceinline__ int select_person(int32_t infector, int32_t loc_off, int32_t loc_count, uint32_t r,
-                                             int type, const int32_t* __restrict__ people, int32_t& victim) {
-    if (loc_count > 1) {
-        int vo = (int)(r % (uint32_t)loc_count);
-        int32_t v = (type == CT_HOME) ? loc_off + vo : people[loc_off + vo];
-        if (v == infector) {
-            vo = (vo + 1) % loc_count;
-            v = (type == CT_HOME) ? loc_off + vo : people[loc_off + vo];
-        }
-        victim = v;
-        return type;
-    }
-    victim = loc_count;
-    return CT_NONE;
-}
-
-struct AgentArrays {
-    const uint8_t* age;
-    const int32_t* household;
-    const int32_t* hh_off;
-    const int32_t* wp_off;
-    const int32_t* wp_people;
-    const int32_t* er_off;     // dense
-    const int32_t* er_people;
-    const uint8_t* st_p;
-    const uint8_t* st_s;
-    const uint8_t* dy_p;
-    const uint8_t* dy_s;
-    const uint8_t* gn_p;
-    const uint8_t* gn_s;
-    uint64_t* claim_p;
-    uint64_t* claim_s;
-    int32_t* list_p;
-    int32_t* list_s;
-    int32_t* trace_victim;     // optional [I*C]
-    uint8_t* trace_type;       // optional [I*C]
-};
-
-// Strain state of an infector (device_processContacts, PandemicSim.cu:3125-3175):
-// profile and shedding day if infectious today, else -1.
-struct Infectivity {
-    int prof_p, dd_p, prof_s, dd_s;
-    uint8_t gen_p, gen_s;
-};
-
-__device__ __forceinline__ Infectivity infectivity(const AgentArrays& A, int32_t idx, int day) {
-    Infectivity f;
-    f.prof_p = f.prof_s = -1;
-    f.dd_p = f.dd_s = 0;
-    f.gen_p = f.gen_s = kGenNone;
-    const uint8_t sp = A.st_p[idx], ss = A.st_s[idx];
-    if (sp >= kInfected) {
-        const int dd = day - (int)A.dy_p[idx];
-        if (dd >= 0 && dd < kCulmination) {
-            f.prof_p = sp - kInfected;
-            f.dd_p = dd;
-            f.gen_p = (uint8_t)(A.gn_p[idx] + 1);
-        }
-    }
-    if (ss >= kInfected) {
-        const int dd = day - (int)A.dy_s[idx];
-        if (dd >= 0 && dd < kCulmination) {
-            f.prof_s = ss - kInfected;
-            f.dd_s = dd;
-            f.gen_s = (uint8_t)(A.gn_s[idx] + 1);
-        }
-    }
-    return f;
-}
-
-// Per-contact draws and claims (device_processContacts PandemicSim.cu:3177-3231
-// with the canonical resolution described at the top of this file).
-template <int C>
-__device__ __forceinline__ void process_contacts(const Model& M, const AgentArrays& A, const Infectivity& f, int kval_sum,
-                                                 const int32_t (&vic)[8], const int (&typ)[8], uint64_t M2,
-                                                 uint32_t pos, bool (&newp)[8], bool (&news)[8]) {
-    const float prob_p = f.prof_p >= 0 ? infection_probability(M, f.prof_p, f.dd_p, 0, kval_sum) : -1.f;
-    const float prob_s = f.prof_s >= 0 ? infection_probability(M, f.prof_s, f.dd_s, 1, kval_sum) : -1.f;
-    uint32_t u[16];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const U4 w = draw4(M2 + (uint64_t)q, M.key0, M.key1);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) u[4 * q + j] = w.w[j];
-    }
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-        const float kv = typ[c] == CT_NONE ? 0.f : 1.f;
-        const int32_t v = vic[c];
-        if (unit_float(u[2 * c]) < prob_p * kv) {
-            if (A.st_p[v] == kSusceptible) {
-                const uint32_t prof = infection_profile(M, u[2 * c] ^ u[(2 * c + 1) % 16]);
-                const uint64_t claim = ((uint64_t)(pos * 8u + (uint32_t)c) << 32) | ((uint32_t)f.gen_p << 8) | prof;
-                const uint64_t old = atomicMin((unsigned long long*)&A.claim_p[v], (unsigned long long)claim);
-                newp[c] = old == ~0ull;
-            }
-        }
-        if (unit_float(u[2 * c + 1]) < prob_s * kv) {
-            if (A.st_s[v] == kSusceptible) {
-                const uint32_t prof = infection_profile(M, u[2 * c + 1] ^ u[(2 * c + 2) % 16]);
-                const uint64_t claim = ((uint64_t)(pos * 8u + (uint32_t)c) << 32) | ((uint32_t)f.gen_s << 8) | prof;
-                const uint64_t old = atomicMin((unsigned long long*)&A.claim_s[v], (unsigned long long)claim);
-                news[c] = old == ~0ull;
-            }
-        }
-    }
-}
-
-__device__ __forceinline__ void append_claims(DayState* st, const AgentArrays& A, const int32_t (&vic)[8],
-                                              const bool (&newp)[8], const bool (&news)[8], int lane) {
-    uint32_t np = 0, ns = 0;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        np += newp[c];
-        ns += news[c];
-    }
-    uint32_t ip = wave_append(&st->n_claimed[0], np, lane);
-    uint32_t is = wave_append(&st->n_claimed[1], ns, lane);
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        if (newp[c]) A.list_p[ip++] = vic[c];
-        if (news[c]) A.list_s[is++] = vic[c];
-    }
-}
-
-// Whether the infector can infect at all today: with alpha*factor <= 0 every
-// draw y >= 0 fails y < p*kval, so contacts need not be generated.
-__device__ __forceinline__ bool can_infect(const Model& M, const Infectivity& f) {
-    const bool p = f.prof_p >= 0 && M.shed[f.prof_p][f.dd_p] * M.factor[0] > 0.f;
-    const bool s = f.prof_s >= 0 && M.shed[f.prof_s][f.dd_s] * M.factor[1] > 0.f;
-    return p || s;
-}
-
-__device__ __forceinline__ void set_mark(uint32_t* __restrict__ mark, int32_t key, int32_t K) {
-    if ((uint32_t)key >= (uint32_t)K) return;
-    const uint32_t bit = 1u << (key & 31);
-    if (!(__hip_atomic_load(&mark[key >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit))
-        atomicOr(&mark[key >> 5], bit);
-}
-
-// Marks the weekday errand buckets infectors draw contacts from: hour h is
-// used iff the errand profile asks for contacts in it (the workplace takes at
-// most 3 of the 5 non-household slots, so the first requested hour always
-// gets one; PandemicSim.cu:1776-1813).
+// Marks the errand buckets today's infectors draw contacts from.
 __global__ __launch_bounds__(256) void k_mark_weekday(const Model* __restrict__ Mp, const DayState* __restrict__ st,
                                                       const int32_t* __restrict__ infected, AgentArrays A,
                                                       uint32_t* __restrict__ mark) {
@@ -852,26 +401,11 @@ __global__ __launch_bounds__(256) void k_mark_weekday(const Model* __restrict__ 
     const int I = st->n_infected;
     const int day = st->day;
     const uint64_t eoff = st->errand_off;
-    const int32_t K = M.wd_K;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < I; pos += stride) {
-        const int32_t idx = infected[pos];
-        const Infectivity f = infectivity(A, idx, day);
-        if (!can_infect(M, f)) continue;
-        const bool adult = A.age[idx] == kAgeAdult;
-        const U4 ue = draw4(eoff + (uint64_t)(idx / 2), M.key0, M.key1);
-        const int rs = 2 * (idx % 2);
-        int32_t d1, d2;
-        weekday_dests(M, ue.w[rs], ue.w[rs + 1], adult, d1, d2);
-        uint32_t k0, k1;
-        weekday_keys(M, d1, d2, k0, k1);
-        const int prof = adult ? (int)((ue.w[rs] ^ ue.w[rs + 1]) % 3u) : 3;
-        if (M.wd_assign[prof][0] > 0) set_mark(mark, (int32_t)k0, K);
-        if (M.wd_assign[prof][1] > 0) set_mark(mark, (int32_t)k1, K);
-    }
+    for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < I; pos += stride)
+        mark_weekday(M, A, day, eoff, infected[pos], mark, 0u);
 }
 
-// Marks the two weekend errand buckets of each infector (PandemicSim.cu:1891-1946).
 __global__ __launch_bounds__(256) void k_mark_weekend(const Model* __restrict__ Mp, const DayState* __restrict__ st,
                                                       const int32_t* __restrict__ infected, AgentArrays A,
                                                       uint32_t* __restrict__ mark) {
@@ -881,32 +415,18 @@ __global__ __launch_bounds__(256) void k_mark_weekend(const Model* __restrict__ 
     const int day = st->day;
     const uint64_t cbase = st->contact_base;
     const uint64_t eoff = st->errand_off;
-    const int32_t K = kWeHours * M.we_E;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < I; pos += stride) {
-        const int32_t idx = infected[pos];
-        const Infectivity f = infectivity(A, idx, day);
-        if (!can_infect(M, f)) continue;
-        const U4 u = draw4(cbase + (uint64_t)(6 * pos), M.key0, M.key1);
-        const int prof = (int)(u.w[3] % 6u);
-        int32_t hour[3], loc[3];
-        weekend_errands(M, eoff + (uint64_t)(int64_t)(2 * idx), hour, loc);
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const int slot = M.we_assign[prof][k];
-            const int hsel = slot == 0 ? hour[0] : (slot == 1 ? hour[1] : hour[2]);
-            const int lsel = slot == 0 ? loc[0] : (slot == 1 ? loc[1] : loc[2]);
-            set_mark(mark, hsel * M.we_E + (lsel - M.we_base), K);
-        }
-    }
+    for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < I; pos += stride)
+        mark_weekend(M, A, day, cbase, eoff, pos, infected[pos], mark, 0u);
 }
 
-// D6w + D7: kernel_doWeekday (PandemicSim.cu:3236-3308) with
-// device_makeContacts_weekday (:1669-1829).
-__global__ __launch_bounds__(kContactThreads) void k_contacts_weekday(const Model* __restrict__ Mp,
-                                                                      DayState* __restrict__ st,
-                                                                      const int32_t* __restrict__ infected,
-                                                                      AgentArrays A, uint64_t workplace_off) {
+// D6w/D6e + D7 over the day's infected list (weekday_contacts,
+// weekend_contacts); claims go to the per-strain claim lists.
+template <bool WEEKEND>
+__global__ __launch_bounds__(kContactThreads) void k_contacts(const Model* __restrict__ Mp, DayState* __restrict__ st,
+                                                              const int32_t* __restrict__ infected, AgentArrays A,
+                                                              int32_t* __restrict__ list_p,
+                                                              int32_t* __restrict__ list_s, uint64_t workplace_off) {
     if (st->done) return;
     const Model& M = *Mp;
     const int I = st->n_infected;
@@ -920,178 +440,14 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts_weekday(const Mode
         int32_t vic[8];
         int typ[8];
         bool newp[8], news[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            vic[c] = -1;
-            typ[c] = CT_NONE;
-            newp[c] = news[c] = false;
-        }
+        clear_contacts(vic, typ, newp, news);
         if (pos < I) {
-            const int32_t idx = infected[pos];
-            const Infectivity f = infectivity(A, idx, day);
-            if (can_infect(M, f) || A.trace_victim) {
-                const uint64_t Mc = cbase + (uint64_t)(6 * pos);
-                const U4 ua = draw4(Mc, M.key0, M.key1);
-                const U4 ub = draw4(Mc + 1, M.key0, M.key1);
-                const uint32_t u[8] = {ua.w[0], ua.w[1], ua.w[2], ua.w[3], ub.w[0], ub.w[1], ub.w[2], ub.w[3]};
-                const uint8_t myage = A.age[idx];
-                const bool adult = myage == kAgeAdult;
-                // household
-                const int32_t hh = A.household[idx];
-                const int32_t hoff = A.hh_off[hh];
-                const int32_t hcnt = A.hh_off[hh + 1] - hoff;
-                // workplace (device_recalcWorkplace PandemicSim.cu:3872-3890)
-                const U4 uw = draw4(workplace_off + (uint64_t)(idx / 4), M.key0, M.key1);
-                uint8_t a2 = myage;
-                const int32_t wp = assign_workplace(M, uw.w[idx % 4], &a2);
-                const int32_t woff = A.wp_off[wp];
-                const int32_t wcnt = A.wp_off[wp + 1] - woff;
-                const int wpd = workplace_max_contacts(M, wp);
-                // errands (device_recalc_weekdayErrandDests_assignProfile :3667-3705)
-                const U4 ue = draw4(eoff + (uint64_t)(idx / 2), M.key0, M.key1);
-                const int rs = 2 * (idx % 2);
-                int32_t d1, d2;
-                weekday_dests(M, ue.w[rs], ue.w[rs + 1], adult, d1, d2);
-                const int prof = adult ? (int)((ue.w[rs] ^ ue.w[rs + 1]) % 3u) : 3;
-                const int h0 = M.wd_assign[prof][0], h1 = M.wd_assign[prof][1];
-                uint32_t k0, k1;
-                weekday_keys(M, d1, d2, k0, k1);
-                const int32_t e0off = A.er_off[k0], e0cnt = A.er_off[k0 + 1] - e0off;
-                const int32_t e1off = A.er_off[k1], e1cnt = A.er_off[k1 + 1] - e1off;
-                const int wtype = adult ? CT_WORKPLACE : CT_SCHOOL;
-                const int etype = adult ? CT_ERRAND : CT_AFTERSCHOOL;
-                int kval = 0;
-#pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    int t;
-                    if (c < 3) {
-                        t = select_person(idx, hoff, hcnt, u[c], CT_HOME, nullptr, vic[c]);
-                    } else if (c < 3 + wpd) {
-                        t = select_person(idx, woff, wcnt, u[c], wtype, A.wp_people, vic[c]);
-                    } else {
-                        const int e = c - 3 - wpd;
-                        if (e < h0)
-                            t = select_person(idx, e0off, e0cnt, u[c], etype, A.er_people, vic[c]);
-                        else if (e < h0 + h1)
-                            t = select_person(idx, e1off, e1cnt, u[c], etype, A.er_people, vic[c]);
-                        else {
-                            vic[c] = -1;
-                            t = CT_NONE;
-                        }
-                    }
-                    typ[c] = t;
-                    kval += t != CT_NONE;
-                }
-                if (A.trace_victim) {
-#pragma unroll
-                    for (int c = 0; c < 8; ++c) {
-                        A.trace_victim[pos * 8 + c] = vic[c];
-                        A.trace_type[pos * 8 + c] = (uint8_t)typ[c];
-                    }
-                }
-                if (can_infect(M, f))
-                    process_contacts<8>(M, A, f, kval, vic, typ, Mc + 2, (uint32_t)pos, newp, news);
-            }
+            if (WEEKEND)
+                weekend_contacts(M, A, day, cbase, eoff, infected[pos], pos, vic, typ, newp, news);
+            else
+                weekday_contacts(M, A, day, cbase, eoff, workplace_off, infected[pos], pos, vic, typ, newp, news);
         }
-        append_claims(st, A, vic, newp, news, lane);
-    }
-}
-
-// D6e + D7: kernel_doWeekend (PandemicSim.cu:3311-3379) with
-// device_makeContacts_weekend (:1833-1950).
-__global__ __launch_bounds__(kContactThreads) void k_contacts_weekend(const Model* __restrict__ Mp,
-                                                                      DayState* __restrict__ st,
-                                                                      const int32_t* __restrict__ infected,
-                                                                      AgentArrays A) {
-    if (st->done) return;
-    const Model& M = *Mp;
-    const int I = st->n_infected;
-    const int day = st->day;
-    const uint64_t cbase = st->contact_base;
-    const uint64_t eoff = st->errand_off;
-    const int lane = threadIdx.x & 63;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < I; b0 += stride) {
-        const int64_t pos = b0 + threadIdx.x;
-        int32_t vic[8];
-        int typ[8];
-        bool newp[8], news[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            vic[c] = -1;
-            typ[c] = CT_NONE;
-            newp[c] = news[c] = false;
-        }
-        if (pos < I) {
-            const int32_t idx = infected[pos];
-            const Infectivity f = infectivity(A, idx, day);
-            if (can_infect(M, f) || A.trace_victim) {
-                const uint64_t Mc = cbase + (uint64_t)(6 * pos);
-                const U4 u = draw4(Mc, M.key0, M.key1);
-                const int32_t hh = A.household[idx];
-                const int32_t hoff = A.hh_off[hh];
-                const int32_t hcnt = A.hh_off[hh + 1] - hoff;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) typ[c] = select_person(idx, hoff, hcnt, u.w[c], CT_HOME, nullptr, vic[c]);
-                const int prof = (int)(u.w[3] % 6u);
-                uint32_t v32[2];
-                draw2x32((uint32_t)(Mc + 1), (uint32_t)M.key0, (uint32_t)M.key1, v32);
-                int32_t hour[3], loc[3];
-                weekend_errands(M, eoff + (uint64_t)(int64_t)(2 * idx), hour, loc);
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    const int slot = M.we_assign[prof][k];
-                    const int hsel = slot == 0 ? hour[0] : (slot == 1 ? hour[1] : hour[2]);
-                    const int lsel = slot == 0 ? loc[0] : (slot == 1 ? loc[1] : loc[2]);
-                    const int32_t key = hsel * M.we_E + (lsel - M.we_base);
-                    const int32_t eoff2 = A.er_off[key], ecnt = A.er_off[key + 1] - eoff2;
-                    typ[3 + k] = select_person(idx, eoff2, ecnt, v32[k], CT_ERRAND, A.er_people, vic[3 + k]);
-                }
-                int kval = 0;
-#pragma unroll
-                for (int c = 0; c < 5; ++c) kval += typ[c] != CT_NONE;
-                if (A.trace_victim) {
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) {
-                        A.trace_victim[pos * 5 + c] = vic[c];
-                        A.trace_type[pos * 5 + c] = (uint8_t)typ[c];
-                    }
-                }
-                if (can_infect(M, f))
-                    process_contacts<5>(M, A, f, kval, vic, typ, Mc + 2, (uint32_t)pos, newp, news);
-            }
-        }
-        append_claims(st, A, vic, newp, news, lane);
-    }
-}
-
-// Winners write status = profile, day = current_day + 1, generation
-// (device_setInfectionStatus PandemicSim.cu:3048-3067); claim words reset.
-__global__ void k_apply(const DayState* __restrict__ st, uint64_t* __restrict__ claim_p,
-                        uint64_t* __restrict__ claim_s, const int32_t* __restrict__ list_p,
-                        const int32_t* __restrict__ list_s, uint8_t* __restrict__ st_p, uint8_t* __restrict__ st_s,
-                        uint8_t* __restrict__ dy_p, uint8_t* __restrict__ dy_s, uint8_t* __restrict__ gn_p,
-                        uint8_t* __restrict__ gn_s) {
-    if (st->done) return;
-    const uint32_t np = st->n_claimed[0], ns = st->n_claimed[1];
-    const uint8_t dset = (uint8_t)(st->day + 1);
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (int64_t)np + ns; i += stride) {
-        if (i < np) {
-            const int32_t v = list_p[i];
-            const uint64_t c = claim_p[v];
-            st_p[v] = (uint8_t)(c & 0xFF);
-            dy_p[v] = dset;
-            gn_p[v] = (uint8_t)((c >> 8) & 0xFF);
-            claim_p[v] = ~0ull;
-        } else {
-            const int32_t v = list_s[i - np];
-            const uint64_t c = claim_s[v];
-            st_s[v] = (uint8_t)(c & 0xFF);
-            dy_s[v] = dset;
-            gn_s[v] = (uint8_t)((c >> 8) & 0xFF);
-            claim_s[v] = ~0ull;
-        }
+        append_claims(st->n_claimed, list_p, list_s, 0, vic, newp, news, lane);
     }
 }
 
@@ -1103,22 +459,6 @@ __global__ void k_end_day(DayState* st) {
     st->skip_full = 1;
     st->skip_subset = 1;
     if (st->day >= st->max_days) st->done = 1;
-}
-
-// Generation histograms (final_countReproduction, PandemicSim.cu:2236-2260).
-__global__ void k_gen_hist(const uint8_t* __restrict__ gn_p, const uint8_t* __restrict__ gn_s, int64_t N,
-                           unsigned long long* __restrict__ hist) {
-    __shared__ uint32_t s_h[512];
-    for (int i = threadIdx.x; i < 512; i += blockDim.x) s_h[i] = 0;
-    __syncthreads();
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
-        atomicAdd(&s_h[gn_p[i]], 1u);
-        atomicAdd(&s_h[256 + gn_s[i]], 1u);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < 512; i += blockDim.x)
-        if (s_h[i]) atomicAdd(&hist[i], (unsigned long long)s_h[i]);
 }
 
 // ============================ host side ============================
@@ -1225,6 +565,89 @@ struct EventTimer : LaunchTimer {
 
 }  // namespace
 
+namespace pt {
+
+int build_model(const pt_params* p, const int32_t seed_key[2], Model& m) {
+    if (p->max_days < 0 || p->max_days > 255)
+        return fail(PT_EINVAL, "max_days %d outside [0,255] (day_t is 8 bits)", p->max_days);
+    if (!(p->people_scaling > 0.f) || !(p->location_scaling > 0.f))
+        return fail(PT_EINVAL, "scaling factors must be positive");
+    if (p->init_pandemic < 0 || p->init_seasonal < 0) return fail(PT_EINVAL, "negative initial infected");
+    const double hh = (double)p->people_scaling * (double)1000000;
+    if (hh > 2.0e8) return fail(PT_EINVAL, "population scale %g too large", (double)p->people_scaling);
+    if ((double)p->location_scaling > 1.0e4) return fail(PT_EINVAL, "location scale too large");
+    m = Model{};
+    base_tables(m);
+    // setup_scaleSimulation (PandemicSim.cu:1127-1162)
+    m.H = round_half_up(hh);
+    int sum = 0;
+    for (int t = 0; t < kTypes; ++t) {
+        const int orig = round_half_up((double)kBaseCounts[t]);
+        const float prod = p->location_scaling * (float)orig;  // float * int
+        int nc = round_half_up((double)prod);
+        if (nc == 0 && orig > 0) nc = 1;
+        m.wp_count[t] = nc;
+        sum += nc;
+    }
+    m.W = sum;
+    int acc = 0;
+    for (int t = 0; t < kTypes; ++t) {
+        m.wp_offset[t] = acc;
+        acc += m.wp_count[t];
+    }
+    // setup_calculateInfectionData (PandemicSim.cu:2828-2840)
+    for (int i = 3; i < 6; ++i)
+        for (int j = 0; j < 10; ++j) m.shed[i][j] = m.shed[i][j] * p->asymp_factor;
+    const float r[2] = {p->r_pandemic, p->r_seasonal};
+    for (int i = 0; i < 2; ++i) {
+        const float d = (1.0f - p->asymp_factor) * p->percent_symptomatic;
+        m.factor[i] = r[i] / d;
+    }
+    m.pct = p->percent_symptomatic;
+    m.key0 = (uint64_t)(int64_t)seed_key[0];
+    m.key1 = (uint64_t)(int64_t)seed_key[1];
+    m.wd_base = m.wp_offset[kTypeAfterschool];
+    m.wd_E = m.W - m.wd_base;
+    m.we_base = m.wp_offset[kFirstErrandRow];
+    m.we_E = m.W - m.we_base;
+    // Lowest weekday destination of an adult: the first errand location, or
+    // offset - 1 of an errand type without locations (fish_weekday_dest).
+    int32_t b1 = m.we_base;
+    for (int t = kFirstErrandRow; t < kTypes; ++t)
+        if (m.wp_count[t] == 0) b1 = std::min(b1, m.wp_offset[t] - 1);
+    m.wd_base1 = std::max(b1, m.wd_base);
+    m.wd_E1 = m.W - m.wd_base1;
+    m.wd_K = m.wd_E + m.wd_E1 + 1;
+    return PT_OK;
+}
+
+void launch_household_types(const Model* d_model, int H, uint8_t* hh_types, unsigned long long* people,
+                            hipStream_t stream) {
+    (void)hipMemsetAsync(hh_types, 0, (size_t)H + 1, stream);
+    k_household_types<<<grid_for(H / 4 + 1, 256, 65535), 256, 0, stream>>>(d_model, hh_types, people);
+}
+
+hipError_t build_population(const Model* d_model, int64_t N, int H, int W, const PopArrays& P, uint32_t* tmp,
+                            TableScratch& ts, uint64_t workplace_off, hipStream_t st) {
+    // K3/K4: household offsets by exclusive scan, then people (PandemicSim.cu:2573-2613).
+    k_household_sizes<<<grid_for((int64_t)H + 1, 256, 65535), 256, 0, st>>>(d_model, P.hh_types, tmp);
+    hipError_t e = exclusive_scan_u32(tmp, (uint32_t*)P.hh_off, (int64_t)H + 1, ts.scan_part, nullptr, st);
+    if (e != hipSuccess) return e;
+    k_generate_households<<<grid_for(H, 256, 65535), 256, 0, st>>>(d_model, P.hh_types, P.hh_off, P.household, P.age);
+    // K5/K6: workplaces, then the workplace table (PandemicSim.cu:3812-3862).
+    k_assign_workplaces<<<grid_for(N / 4 + 1, 256, 65535), 256, 0, st>>>(d_model, workplace_off, P.age, tmp);
+    e = build_table(ts, tmp, nullptr, N, nullptr, W, ID_IDENTITY, (int32_t)N, P.wp_people, P.wp_off, nullptr, st,
+                    nullptr);
+    if (e != hipSuccess) return e;
+    // K7: statuses; claim words start empty.
+    k_init_status<<<grid_for(N, 256, 65535), 256, 0, st>>>(N, P.st_p, P.st_s, P.dy_p, P.dy_s, P.gn_p, P.gn_s);
+    k_fill_u64<<<grid_for(N, 256, 65535), 256, 0, st>>>(P.claim_p, N, ~0ull);
+    k_fill_u64<<<grid_for(N, 256, 65535), 256, 0, st>>>(P.claim_s, N, ~0ull);
+    return hipGetLastError();
+}
+
+}  // namespace pt
+
 struct pt_sim {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -1270,26 +693,6 @@ struct pt_sim {
 };
 
 namespace {
-
-template <class T>
-int dev_alloc(T** p, int64_t count) {
-    if (count <= 0) count = 1;
-    if (hipMalloc((void**)p, (size_t)count * sizeof(T)) != hipSuccess) {
-        *p = nullptr;
-        return fail(PT_ENOMEM, "hipMalloc of %lld bytes failed", (long long)(count * (int64_t)sizeof(T)));
-    }
-    return 0;
-}
-
-template <class T>
-void dev_free(T*& p) {
-    if (p) (void)hipFree((void*)p);
-    p = nullptr;
-}
-
-int grid_for(int64_t n, int threads, int cap) {
-    return (int)std::max<int64_t>(1, std::min<int64_t>((n + threads - 1) / threads, cap));
-}
 
 // Buffers sized by the population (allocated in pt_sim_setup).
 void free_population(pt_sim* s) {
@@ -1346,8 +749,6 @@ AgentArrays agent_arrays(pt_sim* s, int32_t* trace_victim, uint8_t* trace_type) 
     A.gn_s = s->gn_s;
     A.claim_p = s->claim_p;
     A.claim_s = s->claim_s;
-    A.list_p = s->list_p;
-    A.list_s = s->list_s;
     A.trace_victim = trace_victim;
     A.trace_type = trace_type;
     return A;
@@ -1410,18 +811,18 @@ int enqueue_day(pt_sim* s, int d) {
     if (e != hipSuccess) return fail(PT_EDEVICE, "selection scan: %s", hipGetErrorString(e));
     launch(tm, KC_SELECT, k_set_nsel, dim3(1), dim3(1), stc, st, (const uint32_t*)s->sblk_cnt,
            (const uint32_t*)s->sblk_off, nb);
-    launch(tm, KC_SELECT, k_compact_sel, dim3(nb), dim3(256), stc, (const DayState*)st, (const uint32_t*)s->stage_keys,
-           (const int32_t*)s->stage_ids, (const uint32_t*)s->sblk_cnt, (const uint32_t*)s->sblk_off, s->sel_keys,
-           s->sel_ids, weekend ? kSelStageWe : kSelStageWd);
+    launch(tm, KC_SELECT, k_compact_runs, dim3(nb), dim3(256), stc, (const int32_t*)&st->skip_subset,
+           (const uint32_t*)s->stage_keys, (const int32_t*)s->stage_ids, (const uint32_t*)s->sblk_cnt,
+           (const uint32_t*)s->sblk_off, s->sel_keys, s->sel_ids, weekend ? kSelStageWe : kSelStageWd);
     e = build_table(s->tables, s->sel_keys, s->sel_ids, n, &st->n_sel, K, mode, N, s->er_people, s->er_off,
                     &st->skip_subset, stc, tm);
     if (e != hipSuccess) return fail(PT_EDEVICE, "subset table build: %s", hipGetErrorString(e));
     if (weekend)
-        launch(tm, KC_CONTACTS_WEEKEND, k_contacts_weekend, dim3(s->grid_contacts), dim3(kContactThreads), stc, M, st,
-               infected, A);
+        launch(tm, KC_CONTACTS_WEEKEND, k_contacts<true>, dim3(s->grid_contacts), dim3(kContactThreads), stc, M, st,
+               infected, A, s->list_p, s->list_s, s->workplace_off);
     else
-        launch(tm, KC_CONTACTS_WEEKDAY, k_contacts_weekday, dim3(s->grid_contacts), dim3(kContactThreads), stc, M, st,
-               infected, A, s->workplace_off);
+        launch(tm, KC_CONTACTS_WEEKDAY, k_contacts<false>, dim3(s->grid_contacts), dim3(kContactThreads), stc, M, st,
+               infected, A, s->list_p, s->list_s, s->workplace_off);
     launch(tm, KC_APPLY, k_apply, dim3(s->grid_contacts), dim3(256), stc, (const DayState*)st, s->claim_p, s->claim_s,
            (const int32_t*)s->list_p, (const int32_t*)s->list_s, s->st_p, s->st_s, s->dy_p, s->dy_s, s->gn_p, s->gn_s);
     launch(tm, KC_END_DAY, k_end_day, dim3(1), dim3(1), stc, st);
@@ -1532,62 +933,16 @@ int pt_glibc_initial_infected(int32_t core_seed, int32_t n_people, int32_t n, in
 int pt_sim_create(const pt_params* p, const int32_t seed_key[2], int32_t device, pt_sim** out) {
     if (!p || !seed_key || !out) return fail(PT_EINVAL, "null argument");
     *out = nullptr;
-    if (p->max_days < 0 || p->max_days > 255)
-        return fail(PT_EINVAL, "max_days %d outside [0,255] (day_t is 8 bits)", p->max_days);
-    if (!(p->people_scaling > 0.f) || !(p->location_scaling > 0.f))
-        return fail(PT_EINVAL, "scaling factors must be positive");
-    if (p->init_pandemic < 0 || p->init_seasonal < 0) return fail(PT_EINVAL, "negative initial infected");
-    const double hh = (double)p->people_scaling * (double)1000000;
-    if (hh > 2.0e8) return fail(PT_EINVAL, "population scale %g too large", (double)p->people_scaling);
-    if ((double)p->location_scaling > 1.0e4) return fail(PT_EINVAL, "location scale too large");
+    Model m{};
+    const int rc = build_model(p, seed_key, m);
+    if (rc != PT_OK) return rc;
     HIP_TRY(hipSetDevice(device));
     pt_sim* s = new pt_sim();
     s->device = device;
     s->params = *p;
-    Model& m = s->model;
-    base_tables(m);
-    // setup_scaleSimulation (PandemicSim.cu:1127-1162)
-    s->H = round_half_up(hh);
-    int sum = 0;
-    for (int t = 0; t < kTypes; ++t) {
-        const int orig = round_half_up((double)kBaseCounts[t]);
-        const float prod = p->location_scaling * (float)orig;  // float * int
-        int nc = round_half_up((double)prod);
-        if (nc == 0 && orig > 0) nc = 1;
-        m.wp_count[t] = nc;
-        sum += nc;
-    }
-    s->W = sum;
-    int acc = 0;
-    for (int t = 0; t < kTypes; ++t) {
-        m.wp_offset[t] = acc;
-        acc += m.wp_count[t];
-    }
-    // setup_calculateInfectionData (PandemicSim.cu:2828-2840)
-    for (int i = 3; i < 6; ++i)
-        for (int j = 0; j < 10; ++j) m.shed[i][j] = m.shed[i][j] * p->asymp_factor;
-    const float r[2] = {p->r_pandemic, p->r_seasonal};
-    for (int i = 0; i < 2; ++i) {
-        const float d = (1.0f - p->asymp_factor) * p->percent_symptomatic;
-        m.factor[i] = r[i] / d;
-    }
-    m.pct = p->percent_symptomatic;
-    m.key0 = (uint64_t)(int64_t)seed_key[0];
-    m.key1 = (uint64_t)(int64_t)seed_key[1];
-    m.H = s->H;
-    m.W = s->W;
-    m.wd_base = m.wp_offset[kTypeAfterschool];
-    m.wd_E = s->W - m.wd_base;
-    m.we_base = m.wp_offset[kFirstErrandRow];
-    m.we_E = s->W - m.we_base;
-    // Lowest weekday destination of an adult: the first errand location, or
-    // offset - 1 of an errand type without locations (fish_weekday_dest).
-    int32_t b1 = m.we_base;
-    for (int t = kFirstErrandRow; t < kTypes; ++t)
-        if (m.wp_count[t] == 0) b1 = std::min(b1, m.wp_offset[t] - 1);
-    m.wd_base1 = std::max(b1, m.wd_base);
-    m.wd_E1 = s->W - m.wd_base1;
-    m.wd_K = m.wd_E + m.wd_E1 + 1;
+    s->model = m;
+    s->H = m.H;
+    s->W = m.W;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete s;
@@ -1619,11 +974,10 @@ int pt_sim_setup(pt_sim* s, int64_t* n_people_out) {
     hipStream_t st = s->stream;
     // K1/K2: household types and the population size (PandemicSim.cu:3634-3651, :2552-2571).
     if (dev_alloc(&s->hh_types, (int64_t)H + 1)) return PT_ENOMEM;
-    HIP_TRY(hipMemsetAsync(s->hh_types, 0, (size_t)H + 1, st));
     HIP_TRY(hipMemsetAsync(s->d_scratch64, 0, sizeof(unsigned long long), st));
     m.N = 0;
     HIP_TRY(hipMemcpyAsync(s->d_model, &m, sizeof m, hipMemcpyHostToDevice, st));
-    k_household_types<<<grid_for(H / 4 + 1, 256, 65535), 256, 0, st>>>(s->d_model, s->hh_types, s->d_scratch64);
+    launch_household_types(s->d_model, H, s->hh_types, s->d_scratch64, st);
     unsigned long long npeople = 0;
     HIP_TRY(hipMemcpyAsync(&npeople, s->d_scratch64, sizeof npeople, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -1663,28 +1017,14 @@ int pt_sim_setup(pt_sim* s, int64_t* n_people_out) {
         free_population(s);
         return fail(PT_ENOMEM, "table scratch allocation failed");
     }
-    // K3/K4: household offsets by exclusive scan, then people
-    // (PandemicSim.cu:2573-2613).  keys doubles as the size array here.
-    uint32_t* sizes = s->keys;
-    k_household_sizes<<<grid_for((int64_t)H + 1, 256, 65535), 256, 0, st>>>(s->d_model, s->hh_types, sizes);
-    HIP_TRY(exclusive_scan_u32(sizes, (uint32_t*)s->hh_off, (int64_t)H + 1, s->tables.scan_part, nullptr, st));
-    k_generate_households<<<grid_for(H, 256, 65535), 256, 0, st>>>(s->d_model, s->hh_types, s->hh_off, s->household,
-                                                                   s->age);
-    uint64_t R = (uint64_t)(H / 4);
-    // K5/K6: workplaces, then the workplace table (PandemicSim.cu:3812-3862).
-    s->workplace_off = R;
-    k_assign_workplaces<<<grid_for(N / 4 + 1, 256, 65535), 256, 0, st>>>(s->d_model, s->workplace_off, s->age,
-                                                                         s->keys);
-    R += (uint64_t)(N / 4);
-    s->rand_after_setup = R;
-    hipError_t e = build_table(s->tables, s->keys, nullptr, N, nullptr, s->W, ID_IDENTITY, (int32_t)N, s->wp_people,
-                               s->wp_off, nullptr, st, nullptr);
-    if (e != hipSuccess) return fail(PT_EDEVICE, "workplace table: %s", hipGetErrorString(e));
-    // K7: statuses; claim words start empty.
-    k_init_status<<<grid_for(N, 256, 65535), 256, 0, st>>>(N, s->st_p, s->st_s, s->dy_p, s->dy_s, s->gn_p, s->gn_s);
-    k_fill_u64<<<grid_for(N, 256, 65535), 256, 0, st>>>(s->claim_p, N, ~0ull);
-    k_fill_u64<<<grid_for(N, 256, 65535), 256, 0, st>>>(s->claim_s, N, ~0ull);
-    HIP_TRY(hipGetLastError());
+    // K3-K7 (the keys buffer is the scratch; the rand offset after the
+    // households is the workplace counter, PandemicSim.cu:2570-2571, :3817).
+    s->workplace_off = (uint64_t)(H / 4);
+    s->rand_after_setup = s->workplace_off + (uint64_t)(N / 4);
+    const PopArrays P{s->hh_types, s->hh_off, s->household, s->age, s->wp_off, s->wp_people, s->st_p, s->st_s,
+                      s->dy_p, s->dy_s, s->gn_p, s->gn_s, s->claim_p, s->claim_s};
+    hipError_t e = build_population(s->d_model, N, H, s->W, P, s->keys, s->tables, s->workplace_off, st);
+    if (e != hipSuccess) return fail(PT_EDEVICE, "population setup: %s", hipGetErrorString(e));
     HIP_TRY(hipStreamSynchronize(st));
     s->setup_done = true;
     s->seeded = false;
