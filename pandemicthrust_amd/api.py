@@ -25,6 +25,9 @@ EXPORTS = [
     "pt_sim_copy_array", "pt_sim_get_scalars", "pt_sim_get_layout", "pt_sim_set_table_mode", "pt_sim_snapshot",
     "pt_sim_restore", "pt_sim_enable_timing",
     "pt_sim_kernel_times", "pt_sim_kernel_name", "pt_sim_synchronize", "pt_sim_last_run_ms", "pt_sim_destroy",
+    "pt_batch_create", "pt_batch_setup", "pt_batch_seed_infections", "pt_batch_get_info", "pt_batch_run",
+    "pt_batch_generation_offsets", "pt_batch_array_bytes", "pt_batch_copy_array", "pt_batch_set_table_mode",
+    "pt_batch_snapshot", "pt_batch_restore", "pt_batch_synchronize", "pt_batch_last_run_ms", "pt_batch_destroy",
 ]
 
 
@@ -88,6 +91,22 @@ def lib():
         L.pt_sim_last_run_ms.restype = C.c_double
         L.pt_sim_last_run_ms.argtypes = [C.c_void_p]
         L.pt_sim_destroy.argtypes = [C.c_void_p]
+        L.pt_batch_create.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]
+        L.pt_batch_setup.argtypes = [C.c_void_p, C.c_void_p]
+        L.pt_batch_seed_infections.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.pt_batch_get_info.argtypes = [C.c_void_p, C.c_void_p]
+        L.pt_batch_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.pt_batch_generation_offsets.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]
+        L.pt_batch_array_bytes.restype = C.c_int64
+        L.pt_batch_array_bytes.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.pt_batch_copy_array.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64]
+        L.pt_batch_set_table_mode.argtypes = [C.c_void_p, C.c_int32]
+        L.pt_batch_snapshot.argtypes = [C.c_void_p]
+        L.pt_batch_restore.argtypes = [C.c_void_p]
+        L.pt_batch_synchronize.argtypes = [C.c_void_p]
+        L.pt_batch_last_run_ms.restype = C.c_double
+        L.pt_batch_last_run_ms.argtypes = [C.c_void_p]
+        L.pt_batch_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -221,3 +240,95 @@ class PandemicSim:
 
     def last_run_ms(self):
         return float(lib().pt_sim_last_run_ms(self.h))
+
+
+class PandemicBatch:
+    """Many independent simulations (replicates) in lockstep on one GPU
+    (include/pandemic/pt_batch.h).  Replicate r ends exactly where a
+    PandemicSim with rows[r] and core_seeds[r] ends; the reference runs one
+    process per constants row instead (PythonBatcher/write_gpu_constants.py:6-49)."""
+
+    def __init__(self, params_list, core_seeds, device=0):
+        n = len(params_list)
+        if n != len(core_seeds) or n == 0:
+            raise ValueError("one core seed per constants row")
+        self.params = list(params_list)
+        self.core_seeds = [int(s) for s in core_seeds]
+        arr = (Params * n)(*self.params)
+        keys = np.zeros(2 * n, np.int32)
+        for r, s in enumerate(self.core_seeds):
+            sd = glibc_seeds(s)
+            keys[2 * r], keys[2 * r + 1] = sd[0], sd[1]
+        h = C.c_void_p()
+        _check(lib().pt_batch_create(n, arr, keys.ctypes.data, device, C.byref(h)))
+        self.h = h
+        self.n = n
+        self.n_people = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def setup(self):
+        """setupSim plus setup_initialInfected of every replicate; returns the populations."""
+        n_people = np.zeros(self.n, np.int64)
+        _check(lib().pt_batch_setup(self.h, n_people.ctypes.data))
+        self.n_people = n_people
+        ids, n_p, n_s = [], np.zeros(self.n, np.int32), np.zeros(self.n, np.int32)
+        for r, p in enumerate(self.params):
+            ids.append(glibc_initial_infected(self.core_seeds[r], int(n_people[r]), p.init_pandemic + p.init_seasonal))
+            n_p[r], n_s[r] = p.init_pandemic, p.init_seasonal
+        flat = np.ascontiguousarray(np.concatenate(ids) if ids else np.zeros(0, np.int32), dtype=np.int32)
+        _check(lib().pt_batch_seed_infections(self.h, flat.ctypes.data if flat.size else None, n_p.ctypes.data,
+                                              n_s.ctypes.data))
+        return n_people
+
+    def info(self):
+        out = np.zeros(4, np.int64)
+        _check(lib().pt_batch_get_info(self.h, out.ctypes.data))
+        return {"replicates": int(out[0]), "D": int(out[1]), "agents": int(out[2]), "slots": int(out[3])}
+
+    def run(self):
+        """Runs every replicate to its end: lists of (counts[days, 16], infected[days])."""
+        D = self.info()["D"]
+        counts = np.zeros((self.n, D, 16), np.int32)
+        inf = np.zeros((self.n, D), np.int32)
+        days = np.zeros(self.n, np.int32)
+        _check(lib().pt_batch_run(self.h, counts.ctypes.data, inf.ctypes.data, days.ctypes.data))
+        return [(counts[r, :days[r]].copy(), inf[r, :days[r]].copy()) for r in range(self.n)]
+
+    def generation_offsets(self, r):
+        md = self.params[r].max_days
+        cp = np.zeros(md + 1, np.int32)
+        cs = np.zeros(md + 1, np.int32)
+        _check(lib().pt_batch_generation_offsets(self.h, r, cp.ctypes.data, cs.ctypes.data, md + 1))
+        return cp, cs
+
+    def array(self, r, which):
+        nbytes = int(lib().pt_batch_array_bytes(self.h, r, which))
+        if nbytes < 0:
+            raise PTError("array %d of replicate %d is not available" % (which, r))
+        dt = np.uint8 if which in _U8_ARRAYS else np.int32
+        out = np.zeros(nbytes // np.dtype(dt).itemsize, dt)
+        if nbytes:
+            _check(lib().pt_batch_copy_array(self.h, r, which, out.ctypes.data, nbytes))
+        return out
+
+    def set_table_mode(self, mode):
+        _check(lib().pt_batch_set_table_mode(self.h, mode))
+
+    def snapshot(self):
+        _check(lib().pt_batch_snapshot(self.h))
+
+    def restore(self):
+        _check(lib().pt_batch_restore(self.h))
+
+    def synchronize(self):
+        _check(lib().pt_batch_synchronize(self.h))
+
+    def last_run_ms(self):
+        return float(lib().pt_batch_last_run_ms(self.h))

@@ -1,0 +1,125 @@
+"""Replicate batch vs single simulations, bit-exact.
+
+Every replicate of a pt_batch run must end exactly where its own pt_sim run
+(same constants row, same seed) ends: daily counts, infected series, the six
+per-agent state arrays and the generation histograms.  The single-simulation
+engine is itself checked against the CPU oracle (test_gpu_parity.py); two
+replicates are also checked against the oracle directly here.
+"""
+import numpy as np
+import pytest
+
+import oracle_ctypes as O
+
+pytestmark = pytest.mark.gpu
+
+# Diverse replicates: population scale, location scale, R, seeds, initial
+# infected, symptomatic share, asymptomatic factor and max_days all vary;
+# one row has R = 0 (extinction on day 10) and one stops at max_days while
+# still infected.
+ROWS = [
+    (["40", "1.8", "1.3", "10", "10", "0.0425", "0.0425", "1", "0"], 1),
+    (["35", "2.4", "1.1", "5", "30", "0.021", "0.021", "1", "0"], 2),
+    (["60", "1.5", "1.5", "20", "0", "0.03", "0.06", "0.7", "0.5"], 3),
+    (["25", "0", "0", "10", "10", "0.022", "0.022", "1", "0"], 4),
+    (["12", "2.5", "1.3", "40", "40", "0.05", "0.03", "1", "0"], 5),
+    (["45", "1.9", "1.6", "7", "9", "0.0213", "0.0213", "0.9", "0.3"], 6),
+    (["30", "1.8", "1.3", "10", "10", "0.08", "0.08", "1", "0"], 7),
+    (["50", "3.0", "2.0", "1", "1", "0.025", "0.04", "1", "0"], 8),
+]
+
+STATE = ["status_p", "status_s", "day_p", "day_s", "gen_p", "gen_s"]
+
+
+def _batch(rows, mode=0):
+    from pandemicthrust_amd import PandemicBatch, Params
+    b = PandemicBatch([Params.from_row(r) for r, _ in rows], [s for _, s in rows])
+    b.set_table_mode(mode)
+    n = b.setup()
+    return b, n
+
+
+def _solo(row, seed):
+    from pandemicthrust_amd import Params, PandemicSim
+    s = PandemicSim(Params.from_row(row), seed)
+    s.setup_sim()
+    return s
+
+
+def _assert_rep_equals_solo(b, r, res, solo, solo_res, tag):
+    c, inf = res
+    sc, sinf = solo_res
+    assert np.array_equal(inf, sinf), (tag, r, inf[:12], sinf[:12])
+    assert np.array_equal(c, sc), (tag, r)
+    for which, name in enumerate(STATE):
+        g, s = b.array(r, which), solo.array(which)
+        if not np.array_equal(g, s):
+            bad = np.nonzero(g != s)[0]
+            raise AssertionError("%s replicate %d %s differs at %d agents, first %s" % (tag, r, name, len(bad), bad[:5]))
+    bp, bs = b.generation_offsets(r)
+    sp, ss = solo.generation_offsets()
+    assert np.array_equal(bp, sp) and np.array_equal(bs, ss), (tag, r)
+
+
+def test_batch_setup_matches_single_simulations():
+    from pandemicthrust_amd.api import PT_AGE, PT_HH_OFFSETS, PT_HH_TYPES, PT_HOUSEHOLD, PT_WP_OFFSETS, PT_WP_PEOPLE
+    b, n = _batch(ROWS)
+    for r, (row, seed) in enumerate(ROWS):
+        s = _solo(row, seed)
+        assert n[r] == s.n_people
+        for which in (PT_HH_TYPES, PT_HH_OFFSETS, PT_HOUSEHOLD, PT_AGE, PT_WP_OFFSETS, PT_WP_PEOPLE) + tuple(range(6)):
+            assert np.array_equal(b.array(r, which), s.array(which)), (r, which)
+        s.close()
+    b.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_batch_run_matches_single_simulations(mode):
+    b, n = _batch(ROWS, mode)
+    res = b.run()
+    for r, (row, seed) in enumerate(ROWS):
+        s = _solo(row, seed)
+        s.set_table_mode(mode)
+        sres = s.run()
+        _assert_rep_equals_solo(b, r, res[r], s, sres, "mode %d" % mode)
+        s.close()
+    # R = 0: only the initial infected, recovered on day 10.
+    c, inf = res[3]
+    assert len(inf) == 11 and inf[-1] == 0 and inf[0] == 20
+    b.close()
+
+
+def test_batch_replicates_match_oracle():
+    b, n = _batch(ROWS[:2])
+    res = b.run()
+    for r, (row, seed) in enumerate(ROWS[:2]):
+        ref = O.OracleSim(O.params_from_row(row), seed)
+        ref.setup()
+        assert ref.n_people == n[r]
+        counts, inf = ref.run()
+        assert np.array_equal(res[r][1], inf) and np.array_equal(res[r][0], counts), r
+        for which, name in enumerate(STATE):
+            assert np.array_equal(b.array(r, which), ref.array(which)), (r, name)
+    b.close()
+
+
+def test_batch_rerun_after_restore_is_identical():
+    b, n = _batch(ROWS[:4])
+    b.snapshot()
+    first = b.run()
+    st1 = [b.array(r, 0).copy() for r in range(4)]
+    b.restore()
+    second = b.run()
+    for r in range(4):
+        assert np.array_equal(first[r][0], second[r][0]) and np.array_equal(first[r][1], second[r][1])
+        assert np.array_equal(st1[r], b.array(r, 0))
+    b.close()
+
+
+def test_batch_of_one_equals_single_simulation():
+    b, n = _batch(ROWS[6:7])
+    res = b.run()
+    s = _solo(*ROWS[6])
+    _assert_rep_equals_solo(b, 0, res[0], s, s.run(), "single")
+    s.close()
+    b.close()
