@@ -26,6 +26,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -103,12 +105,54 @@ def cpu_baseline():
                       "(%d agents, %.1f s loop)" % (r["setup_s"], r["days"], r["n_people"], r["loop_s"])}
 
 
+def replicate_bench(n_reps):
+    """256-scenario ensemble (pandemicthrust_amd/data/config2_scenarios.csv),
+    each replicate run to its end (180 days or extinction), setup included:
+    plain = one single-simulation handle after another; batched = one batch.
+    Every replicate's daily counts must agree between the two."""
+    from pandemicthrust_amd import PandemicBatch, PandemicSim, Params
+    from pandemicthrust_amd.scenarios import load_config2
+    rows = load_config2()[:n_reps]
+    params = [Params.from_row(f) for _, f in rows]
+    seeds = [s for s, _ in rows]
+    t0 = time.perf_counter()
+    b = PandemicBatch(params, seeds)
+    n_people = b.setup()
+    res = b.run()
+    batched_s = time.perf_counter() - t0
+    batched_loop_ms = b.last_run_ms()
+    info = b.info()
+    b.close()
+    print("replicates: batched %d in %.3f s (loop %.1f ms)" % (len(rows), batched_s, batched_loop_ms),
+          file=sys.stderr, flush=True)
+    identical = True
+    t0 = time.perf_counter()
+    for r, (p, s) in enumerate(zip(params, seeds)):
+        sim = PandemicSim(p, s)
+        sim.setup_sim()
+        counts, inf = sim.run()
+        sim.close()
+        identical &= bool(np.array_equal(counts, res[r][0]) and np.array_equal(inf, res[r][1]))
+    plain_s = time.perf_counter() - t0
+    print("replicates: plain %d in %.3f s" % (len(rows), plain_s), file=sys.stderr, flush=True)
+    person_days = int(sum(int(n_people[r]) * len(res[r][1]) for r in range(len(rows))))
+    return {"workload": "%d scenarios of pandemicthrust_amd/data/config2_scenarios.csv (N 50k-2M log-uniform, "
+                        "max_days 180), setup + run to the end" % len(rows),
+            "agents": info["agents"], "person_days": person_days,
+            "batched_per_s": round(len(rows) / batched_s, 2), "plain_per_s": round(len(rows) / plain_s, 2),
+            "speedup": round(plain_s / batched_s, 2), "batched_s": round(batched_s, 3),
+            "batched_loop_ms": round(batched_loop_ms, 1), "plain_s": round(plain_s, 3),
+            "batched_person_days_per_s": round(person_days / (batched_loop_ms * 1e-3), 1),
+            "identical_results": identical}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--replicates", type=int, default=256, help="scenarios in the replicate leg (0 skips it)")
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -206,10 +250,12 @@ def main():
                           "frac": round(loop_bytes * args.steps / loop_s / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels": table,
     }
+    sim.close()
+    if args.replicates > 0:
+        line["replicates"] = replicate_bench(args.replicates)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
-    sim.close()
     return 0
 
 
