@@ -41,10 +41,17 @@ CONSTANTS_HEADER = ("constants,max_days,reproduction_number_pandemic,reproductio
                     "location_scaling_factor,percent_symptomatic,asymp_factor")
 
 
-def algorithmic_bytes(N, W, lay, inf, counts):
+def algorithmic_bytes(N, W, lay, inf, counts, full, pairs):
     """Algorithmic bytes per kernel class for one loop, and the loop total of
     the per-day model B_wd = 43N + 89I + 3X + 8W + 4, B_we = 62N + 62I + 3X +
-    40W + 4 (terminal I = 0 day: 2N)."""
+    40W + 4 (terminal I = 0 day: 2N).
+
+    full[d], pairs[d]: the day's errand table (every bucket, n = 2N or 3N
+    pairs; or the queried buckets only, pairs[d] selected pairs) as the
+    engine logged it.  Full-table days account to gen/sort/offsets, subset
+    days to select/subset_sort.  Per pair and radix pass: upsweep reads the
+    key (4 B); downsweep reads key (+ id when ids are explicit or after the
+    first pass) and writes key and id (12 or 16 B)."""
     per = {}
 
     def add(k, b):
@@ -65,24 +72,27 @@ def algorithmic_bytes(N, W, lay, inf, counts):
             X = (int(counts[d][1]) - int(counts[d + 1][1])) + (int(counts[d][9]) - int(counts[d + 1][9]))
         else:
             X = 0
+        passes = lay["we_passes"] if weekend else lay["wd_passes"]
+        buckets = lay["we_buckets"] if weekend else lay["wd_buckets"]
         if weekend:
-            n = 3 * N
-            passes = lay["we_passes"]
-            add("gen_weekend", 4 * n)
             add("contacts_weekend", 56 * I)
             loop += 62 * N + 62 * I + 3 * X + 40 * W + 4
-            buckets = lay["we_buckets"]
         else:
-            n = 2 * N
-            passes = lay["wd_passes"]
-            add("gen_weekday", N + 4 * n)
             add("contacts_weekday", 83 * I)
             loop += 43 * N + 89 * I + 3 * X + 8 * W + 4
-            buckets = lay["wd_buckets"]
-        for p in range(passes):
-            add("sort_upsweep", 4 * n)
-            add("sort_downsweep", n * (4 + (4 if p else 0) + 4 + 4))
-        add("offsets", 4 * n + 4 * (buckets + 1))
+        if d < len(full) and full[d]:
+            n = 3 * N if weekend else 2 * N
+            add("gen_weekend" if weekend else "gen_weekday", 4 * n + (0 if weekend else N))
+            for p in range(passes):
+                add("sort_upsweep", 4 * n)
+                add("sort_downsweep", n * (4 + (4 if p else 0) + 4 + 4))
+            add("offsets", 4 * n + 4 * (buckets + 1))
+        elif d < len(full):
+            m = int(pairs[d])
+            add("select", (0 if weekend else N) + 8 * m + 16 * m)   # ages; staged pairs; compaction
+            for p in range(passes):
+                add("subset_sort", 4 * m + 16 * m)
+            add("subset_sort", 4 * m + 4 * (buckets + 1))
         add("apply", 3 * X + 16 * X)
     return per, loop
 
@@ -206,7 +216,8 @@ def main():
 
     days_per_step = days_total / max(args.steps, 1)
     value = N * days_total / wall
-    per_loop, loop_bytes = algorithmic_bytes(N, W, lay, inf, counts)
+    full, pairs = sim.table_log()
+    per_loop, loop_bytes = algorithmic_bytes(N, W, lay, inf, counts, full, pairs)
 
     # Per-kernel table from the profiled warmup loop (one loop's worth).
     table = {}

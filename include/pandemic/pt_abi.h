@@ -126,6 +126,12 @@ int pt_sim_get_layout(pt_sim* sim, int64_t out[8]);
  * hooks show complete tables only in mode 1. */
 int pt_sim_set_table_mode(pt_sim* sim, int32_t mode);
 
+/* Measurement hook: the errand table of each day run since setup / restore.
+ * full_out[d] = 1 if day d built every bucket, 0 if only the queried ones;
+ * pairs_out[d] = pairs sorted that day.  Fills min(len, days run) entries
+ * and returns their number (either pointer may be NULL). */
+int pt_sim_table_log(pt_sim* sim, int32_t* full_out, int32_t* pairs_out, int32_t len);
+
 /* Saves the post-setup, post-seeding state on the device; restore brings
  * the simulation back to it (used to repeat the daily loop). */
 int pt_sim_snapshot(pt_sim* sim);

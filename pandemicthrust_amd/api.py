@@ -22,7 +22,8 @@ _U8_ARRAYS = {PT_STATUS_P, PT_STATUS_S, PT_DAY_P, PT_DAY_S, PT_GEN_P, PT_GEN_S, 
 EXPORTS = [
     "pt_last_error", "pt_glibc_seeds", "pt_glibc_initial_infected", "pt_sim_create", "pt_sim_setup",
     "pt_sim_seed_infections", "pt_sim_step", "pt_sim_run", "pt_sim_generation_offsets", "pt_sim_array_bytes",
-    "pt_sim_copy_array", "pt_sim_get_scalars", "pt_sim_get_layout", "pt_sim_set_table_mode", "pt_sim_snapshot",
+    "pt_sim_copy_array", "pt_sim_get_scalars", "pt_sim_get_layout", "pt_sim_set_table_mode", "pt_sim_table_log",
+    "pt_sim_snapshot",
     "pt_sim_restore", "pt_sim_enable_timing",
     "pt_sim_kernel_times", "pt_sim_kernel_name", "pt_sim_synchronize", "pt_sim_last_run_ms", "pt_sim_destroy",
     "pt_batch_create", "pt_batch_setup", "pt_batch_seed_infections", "pt_batch_get_info", "pt_batch_run",
@@ -81,6 +82,7 @@ def lib():
         L.pt_sim_get_scalars.argtypes = [C.c_void_p, C.c_void_p]
         L.pt_sim_get_layout.argtypes = [C.c_void_p, C.c_void_p]
         L.pt_sim_set_table_mode.argtypes = [C.c_void_p, C.c_int32]
+        L.pt_sim_table_log.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
         L.pt_sim_snapshot.argtypes = [C.c_void_p]
         L.pt_sim_restore.argtypes = [C.c_void_p]
         L.pt_sim_enable_timing.argtypes = [C.c_void_p, C.c_int32]
@@ -207,6 +209,13 @@ class PandemicSim:
     def set_table_mode(self, mode):
         """0 automatic, 1 full errand tables every day, 2 queried buckets only."""
         _check(lib().pt_sim_set_table_mode(self.h, mode))
+
+    def table_log(self):
+        """Per day since setup / restore: (full table?, pairs sorted)."""
+        full = np.zeros(256, np.int32)
+        pairs = np.zeros(256, np.int32)
+        n = _check(lib().pt_sim_table_log(self.h, full.ctypes.data, pairs.ctypes.data, 256))
+        return full[:n].astype(bool), pairs[:n].astype(np.int64)
 
     def snapshot(self):
         _check(lib().pt_sim_snapshot(self.h))

@@ -325,6 +325,10 @@ __global__ __launch_bounds__(kRepThreads) void kb_decide(const DayState* __restr
     }
 }
 
+// Selection, first pass.  A block of a replicate that takes every pair today
+// (skip_full == 0) only records its pair count; a block of a replicate that
+// takes its queried buckets stages those pairs (select_block) and records
+// their count.
 template <int PER, bool WEEKEND>
 __global__ __launch_bounds__(kSelThreads) void kb_select(const DayState* __restrict__ g, DayState* __restrict__ sts,
                                                          const RepInfo* __restrict__ reps,
@@ -334,8 +338,9 @@ __global__ __launch_bounds__(kSelThreads) void kb_select(const DayState* __restr
                                                          uint32_t* __restrict__ stage_keys, int32_t* __restrict__ stage_ids,
                                                          uint32_t* __restrict__ blk_cnt) {
     if (g->done) return;
+    constexpr int KPA = WEEKEND ? 3 : 2;
     constexpr int AG = kSelThreads * PER;
-    constexpr int STAGE = AG * (WEEKEND ? 3 : 2);
+    constexpr int STAGE = AG * KPA;
     const int64_t b = blockIdx.x;
     const int64_t a = b * AG;
     const int r = tile_rep[a / kSweepTile];
@@ -346,10 +351,52 @@ __global__ __launch_bounds__(kSelThreads) void kb_select(const DayState* __restr
         if (threadIdx.x == 0) blk_cnt[b] = 0;
         return;
     }
+    if (st->skip_full == 0) {
+        if (threadIdx.x == 0) blk_cnt[b] = (uint32_t)(KPA * min((int64_t)AG, (int64_t)ri.N - la));
+        return;
+    }
     if (!select_block<PER, WEEKEND>(models[r], st->errand_off, age + ri.abase, mark, WEEKEND ? ri.kb_we : ri.kb_wd,
-                                    st->skip_full == 0, la, ri.N, stage_keys + b * STAGE, stage_ids + b * STAGE,
-                                    blk_cnt + b))
+                                    false, la, ri.N, stage_keys + b * STAGE, stage_ids + b * STAGE, blk_cnt + b))
         st->fault = 1;
+}
+
+// Selection, second pass: every block writes its pairs at its offset in the
+// day's pair list — regenerated for every-pair replicates, copied from the
+// staging area for queried-bucket replicates.
+template <int PER, bool WEEKEND>
+__global__ __launch_bounds__(kSelThreads) void kb_emit(const DayState* __restrict__ g, DayState* __restrict__ sts,
+                                                       const RepInfo* __restrict__ reps, const Model* __restrict__ models,
+                                                       const int32_t* __restrict__ tile_rep,
+                                                       const uint8_t* __restrict__ age, const uint32_t* __restrict__ mark,
+                                                       const uint32_t* __restrict__ stage_keys,
+                                                       const int32_t* __restrict__ stage_ids,
+                                                       const uint32_t* __restrict__ blk_cnt,
+                                                       const uint32_t* __restrict__ blk_off, uint32_t* __restrict__ sel_keys,
+                                                       int32_t* __restrict__ sel_ids) {
+    if (g->done) return;
+    constexpr int KPA = WEEKEND ? 3 : 2;
+    constexpr int AG = kSelThreads * PER;
+    constexpr int STAGE = AG * KPA;
+    const int64_t b = blockIdx.x;
+    const int64_t a = b * AG;
+    const int r = tile_rep[a / kSweepTile];
+    DayState* st = sts + r;
+    const RepInfo ri = reps[r];
+    const int64_t la = a - ri.abase;
+    if (st->done || la >= ri.N) return;
+    const uint32_t off = blk_off[b];
+    if (st->skip_full == 0) {
+        if (!select_block<PER, WEEKEND>(models[r], st->errand_off, age + ri.abase, mark,
+                                        WEEKEND ? ri.kb_we : ri.kb_wd, true, la, ri.N, sel_keys + off, sel_ids + off,
+                                        nullptr))
+            st->fault = 1;
+        return;
+    }
+    const uint32_t n = blk_cnt[b];
+    for (uint32_t i = threadIdx.x; i < n; i += kSelThreads) {
+        sel_keys[off + i] = stage_keys[b * STAGE + i];
+        sel_ids[off + i] = stage_ids[b * STAGE + i];
+    }
 }
 
 __global__ void kb_set_nsel(DayState* __restrict__ g, const uint32_t* __restrict__ blk_cnt,
@@ -573,8 +620,14 @@ int enqueue_batch_day(pt_batch* b, int d) {
     hipError_t e = exclusive_scan_u32(b->sblk_cnt, b->sblk_off, nb, b->tables.scan_part, &g->done, s);
     if (e != hipSuccess) return fail(PT_EDEVICE, "selection scan: %s", hipGetErrorString(e));
     kb_set_nsel<<<1, 1, 0, s>>>(g, b->sblk_cnt, b->sblk_off, nb);
-    k_compact_runs<<<nb, 256, 0, s>>>(&g->done, b->stage_keys, b->stage_ids, b->sblk_cnt, b->sblk_off, b->sel_keys,
-                                      b->sel_ids, weekend ? kSelStageWe : kSelStageWd);
+    if (weekend)
+        kb_emit<kSelPerWe, true><<<nb, kSelThreads, 0, s>>>(g, b->d_states, b->d_reps, b->d_models, b->tile_rep, b->age,
+                                                             b->mark, b->stage_keys, b->stage_ids, b->sblk_cnt, b->sblk_off,
+                                                             b->sel_keys, b->sel_ids);
+    else
+        kb_emit<kSelPerWd, false><<<nb, kSelThreads, 0, s>>>(g, b->d_states, b->d_reps, b->d_models, b->tile_rep,
+                                                              b->age, b->mark, b->stage_keys, b->stage_ids, b->sblk_cnt,
+                                                              b->sblk_off, b->sel_keys, b->sel_ids);
     const int64_t n_cap = (weekend ? 3 : 2) * b->P;
     e = build_table(b->tables, b->sel_keys, b->sel_ids, n_cap, &g->n_sel, (int32_t)K, ID_IDENTITY, 0, b->er_people,
                     b->er_off, &g->done, s, nullptr);
@@ -740,6 +793,7 @@ int pt_batch_setup(pt_batch* b, int64_t* n_people_out) {
     HIP_TRY(hipMemsetAsync(b->st_p, 0, (size_t)P, st));
     HIP_TRY(hipMemsetAsync(b->st_s, 0, (size_t)P, st));
     // K3-K7 of every replicate into its slices (sel_keys is the scratch).
+    b->tables.grouped_keys = false;
     for (int r = 0; r < R; ++r) {
         const RepInfo& ri = b->reps[r];
         const int64_t a = ri.abase;
@@ -751,6 +805,8 @@ int pt_batch_setup(pt_batch* b, int64_t* n_people_out) {
         if (e != hipSuccess) return fail(PT_EDEVICE, "replicate %d setup: %s", r, hipGetErrorString(e));
     }
     HIP_TRY(hipStreamSynchronize(st));
+    // Day tables hold the replicates' key ranges side by side.
+    b->tables.grouped_keys = true;
     b->setup_done = true;
     if (reset_batch_state(b)) return PT_EDEVICE;
     if (n_people_out)

@@ -297,11 +297,13 @@ __device__ __forceinline__ bool is_marked(const uint32_t* __restrict__ mark, uin
 
 // One selection block (kSelThreads threads) over the agents from blk_a0 of a
 // population of N: regenerates each agent's errand keys (weekday_keys or
-// hour * E + location - base) and stages, in ascending agent order, the pairs
-// (kbias + key, agent) whose bucket kbias + key is marked, or every pair when
-// all is set.  Staging is block-ordered, so after compaction the ids ascend
-// within every bucket (equal buckets never come from two hours of one
-// agent).  Returns false if a destination fell outside the layout.
+// hour * E + location - base) and writes to sk/si, in ascending agent order,
+// the pairs (kbias + key, agent) whose bucket kbias + key is marked, or every
+// pair when all is set (through LDS, so the global writes are contiguous).
+// Block-ordered output keeps the ids ascending within every bucket (equal
+// buckets never come from two hours of one agent).  *cnt_out (if set) gets
+// the number written.  Returns false if a destination fell outside the
+// layout.
 template <int PER, bool WEEKEND>
 __device__ __forceinline__ bool select_block(const Model& M, uint64_t eoff, const uint8_t* __restrict__ age,
                                              const uint32_t* __restrict__ mark, uint32_t kbias, bool all,
@@ -373,14 +375,21 @@ __device__ __forceinline__ bool select_block(const Model& M, uint64_t eoff, cons
         total += s_w[w];
     }
     pos += inc - cnt;
+    __shared__ uint32_t s_k[kSelThreads * NK];
+    __shared__ int32_t s_i[kSelThreads * NK];
 #pragma unroll
     for (int i = 0; i < NK; ++i)
         if (sel & (1u << i)) {
-            sk[pos] = key[i];
-            si[pos] = (int32_t)(a0 + i / KPA);
+            s_k[pos] = key[i];
+            s_i[pos] = (int32_t)(a0 + i / KPA);
             pos++;
         }
-    if (threadIdx.x == 0) *cnt_out = total;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < total; j += kSelThreads) {
+        sk[j] = s_k[j];
+        si[j] = s_i[j];
+    }
+    if (cnt_out && threadIdx.x == 0) *cnt_out = total;
     return !bad;
 }
 

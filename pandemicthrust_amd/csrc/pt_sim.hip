@@ -451,8 +451,14 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts(const Model* __res
     }
 }
 
-__global__ void k_end_day(DayState* st) {
+// Also logs the day's errand table: full (every pair, n_full) or the subset
+// of queried buckets (n_sel pairs).
+__global__ void k_end_day(DayState* st, int32_t* __restrict__ tab_full, int32_t* __restrict__ tab_pairs,
+                          int32_t n_full) {
     if (st->done) return;
+    const int d = st->day;
+    tab_full[d] = st->skip_full == 0 ? 1 : 0;
+    tab_pairs[d] = st->skip_full == 0 ? n_full : (st->skip_subset == 0 ? (int32_t)st->n_sel : 0);
     st->day += 1;
     st->n_claimed[0] = 0;
     st->n_claimed[1] = 0;
@@ -514,7 +520,7 @@ int round_half_up(double d) { return (int)floor(d + 0.5); }  // roundHalfUp_toIn
 const char* kKernelNames[KC_COUNT] = {"sweep",         "plan",       "compact",   "gen_weekday",
                                       "gen_weekend",   "sort_upsweep", "sort_scan", "sort_downsweep",
                                       "offsets",       "contacts_weekday", "contacts_weekend", "apply",
-                                      "end_day",       "mark",       "select",    "setup"};
+                                      "end_day",       "mark",       "select",    "subset_sort", "setup"};
 
 // Per-class kernel timing.  mode 0: off; -1: every class; k+1: class k only.
 struct EventTimer : LaunchTimer {
@@ -681,6 +687,7 @@ struct pt_sim {
     int nblk_sweep = 0;
     DayState* d_state = nullptr;
     int32_t *d_counts = nullptr, *d_inf = nullptr;  // [256*16], [256]
+    int32_t *d_tab_full = nullptr, *d_tab_pairs = nullptr;  // [256] errand-table log per day
     unsigned long long* d_scratch64 = nullptr;      // 512
     TableScratch tables;
     int host_day = 0;      // days enqueued since setup / restore
@@ -814,8 +821,9 @@ int enqueue_day(pt_sim* s, int d) {
     launch(tm, KC_SELECT, k_compact_runs, dim3(nb), dim3(256), stc, (const int32_t*)&st->skip_subset,
            (const uint32_t*)s->stage_keys, (const int32_t*)s->stage_ids, (const uint32_t*)s->sblk_cnt,
            (const uint32_t*)s->sblk_off, s->sel_keys, s->sel_ids, weekend ? kSelStageWe : kSelStageWd);
+    ClassTimer subset_tm(tm, KC_SUBSET_SORT);
     e = build_table(s->tables, s->sel_keys, s->sel_ids, n, &st->n_sel, K, mode, N, s->er_people, s->er_off,
-                    &st->skip_subset, stc, tm);
+                    &st->skip_subset, stc, &subset_tm);
     if (e != hipSuccess) return fail(PT_EDEVICE, "subset table build: %s", hipGetErrorString(e));
     if (weekend)
         launch(tm, KC_CONTACTS_WEEKEND, k_contacts<true>, dim3(s->grid_contacts), dim3(kContactThreads), stc, M, st,
@@ -825,7 +833,7 @@ int enqueue_day(pt_sim* s, int d) {
                infected, A, s->list_p, s->list_s, s->workplace_off);
     launch(tm, KC_APPLY, k_apply, dim3(s->grid_contacts), dim3(256), stc, (const DayState*)st, s->claim_p, s->claim_s,
            (const int32_t*)s->list_p, (const int32_t*)s->list_s, s->st_p, s->st_s, s->dy_p, s->dy_s, s->gn_p, s->gn_s);
-    launch(tm, KC_END_DAY, k_end_day, dim3(1), dim3(1), stc, st);
+    launch(tm, KC_END_DAY, k_end_day, dim3(1), dim3(1), stc, st, s->d_tab_full, s->d_tab_pairs, (int32_t)n);
     e = hipGetLastError();
     if (e != hipSuccess) return fail(PT_EDEVICE, "day launch: %s", hipGetErrorString(e));
     s->last_weekend = weekend ? 1 : 0;
@@ -954,7 +962,8 @@ int pt_sim_create(const pt_params* p, const int32_t seed_key[2], int32_t device,
         s->grid_gen = cus * 16;
     }
     if (dev_alloc(&s->d_model, 1) || dev_alloc(&s->d_state, 1) || dev_alloc(&s->d_counts, 256 * 16) ||
-        dev_alloc(&s->d_inf, 256) || dev_alloc(&s->d_scratch64, 512)) {
+        dev_alloc(&s->d_inf, 256) || dev_alloc(&s->d_scratch64, 512) || dev_alloc(&s->d_tab_full, 256) ||
+        dev_alloc(&s->d_tab_pairs, 256)) {
         const std::string msg = g_err;
         pt_sim_destroy(s);
         return fail(PT_ENOMEM, "%s", msg.c_str());
@@ -1275,6 +1284,20 @@ int pt_sim_set_table_mode(pt_sim* s, int32_t mode) {
     return PT_OK;
 }
 
+int pt_sim_table_log(pt_sim* s, int32_t* full_out, int32_t* pairs_out, int32_t len) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    if (!s->setup_done) return fail(PT_ESTATE, "table log before setup");
+    HIP_TRY(hipSetDevice(s->device));
+    DayState h;
+    if (read_state(s, &h)) return PT_EDEVICE;
+    // The terminal I == 0 day builds no table and is not logged.
+    const int days = std::max(0, std::min(len, h.done && h.n_infected == 0 ? h.days_run - 1 : h.days_run));
+    if (days > 0 && full_out) HIP_TRY(hipMemcpy(full_out, s->d_tab_full, sizeof(int32_t) * days, hipMemcpyDeviceToHost));
+    if (days > 0 && pairs_out)
+        HIP_TRY(hipMemcpy(pairs_out, s->d_tab_pairs, sizeof(int32_t) * days, hipMemcpyDeviceToHost));
+    return days;
+}
+
 int pt_sim_enable_timing(pt_sim* s, int32_t enable) {
     if (!s) return fail(PT_EINVAL, "null handle");
     s->timer.reset();
@@ -1311,6 +1334,8 @@ void pt_sim_destroy(pt_sim* s) {
     dev_free(s->d_state);
     dev_free(s->d_counts);
     dev_free(s->d_inf);
+    dev_free(s->d_tab_full);
+    dev_free(s->d_tab_pairs);
     dev_free(s->d_scratch64);
     if (s->run_begin) (void)hipEventDestroy(s->run_begin);
     if (s->run_end) (void)hipEventDestroy(s->run_end);

@@ -31,6 +31,7 @@ enum KernelClass : int {
     KC_END_DAY,
     KC_MARK,
     KC_SELECT,
+    KC_SUBSET_SORT,
     KC_SETUP,
     KC_COUNT
 };
@@ -43,6 +44,15 @@ struct LaunchTimer {
     // A fresh (start, stop) event pair accounted to class cls.
     virtual void events(int cls, hipEvent_t* start, hipEvent_t* stop) = 0;
     virtual ~LaunchTimer() {}
+};
+
+// Accounts every launch to one class of another timer.
+struct ClassTimer : LaunchTimer {
+    LaunchTimer* inner;
+    int cls;
+    ClassTimer(LaunchTimer* t, int c) : inner(t), cls(c) {}
+    bool want(int) const override { return inner && inner->want(cls); }
+    void events(int, hipEvent_t* start, hipEvent_t* stop) override { inner->events(cls, start, stop); }
 };
 
 template <typename F, typename... Args>
@@ -70,6 +80,9 @@ struct TableScratch {
     uint32_t* hist = nullptr;    // 256 * kMaxSortBlocks
     uint32_t* scan_part = nullptr;
     int64_t n_cap = 0;
+    // Keys come in long runs of equal high bits (tables of many simulations
+    // side by side): count digits with wave-aggregated atomics.
+    bool grouped_keys = false;
 };
 
 int table_scratch_alloc(TableScratch& s, int64_t n_cap);

@@ -14,9 +14,15 @@ namespace pt {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kItems = 16;
+#ifndef PT_SORT_ITEMS
+#define PT_SORT_ITEMS 16
+#endif
+constexpr int kItems = PT_SORT_ITEMS;  // pairs per thread per tile
 constexpr int kTile = kThreads * kItems;  // 4096 pairs per tile
-constexpr int kMaxSortBlocks = 1024;
+#ifndef PT_SORT_BLOCKS
+#define PT_SORT_BLOCKS 1024
+#endif
+constexpr int kMaxSortBlocks = PT_SORT_BLOCKS;
 constexpr int kMaxBitsSupported = 11;
 constexpr int kScanSeg = 8192;  // elements per scan block
 constexpr int kScanItems = kScanSeg / kThreads;
@@ -69,7 +75,7 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_w
 }
 
 // Per-block digit histogram of the block's chunk.
-template <int BITS>
+template <int BITS, bool MATCH>
 __global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t* __restrict__ keys, int64_t n_host,
                                                       const uint32_t* __restrict__ n_dev, int shift,
                                                       uint32_t* __restrict__ hist, int nblk, const int32_t* done) {
@@ -82,15 +88,48 @@ __global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t* __restrict
     int64_t begin, end;
     block_range(n_host, n_dev, begin, end);
     const uint4* k4 = reinterpret_cast<const uint4*>(keys);
-    for (int64_t q = begin / 4 + tid; q * 4 < end; q += kThreads) {
-        if (q * 4 + 3 < end) {
-            const uint4 v = k4[q];
-            atomicAdd(&s_h[wave][(v.x >> shift) & (RADIX - 1)], 1u);
-            atomicAdd(&s_h[wave][(v.y >> shift) & (RADIX - 1)], 1u);
-            atomicAdd(&s_h[wave][(v.z >> shift) & (RADIX - 1)], 1u);
-            atomicAdd(&s_h[wave][(v.w >> shift) & (RADIX - 1)], 1u);
-        } else {
-            for (int64_t j = q * 4; j < end; ++j) atomicAdd(&s_h[wave][(keys[j] >> shift) & (RADIX - 1)], 1u);
+    if (MATCH) {
+        // Keys with long runs of one digit (e.g. grouped by simulation): when
+        // every active lane of the wave holds the same digit, one lane adds
+        // the count; otherwise each lane adds its own.
+        for (int64_t q = begin / 4 + tid; q * 4 < end; q += kThreads) {
+            uint32_t kk[4];
+            if (q * 4 + 3 < end) {
+                const uint4 v = k4[q];
+                kk[0] = v.x;
+                kk[1] = v.y;
+                kk[2] = v.z;
+                kk[3] = v.w;
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) kk[c] = q * 4 + c < end ? keys[q * 4 + c] : 0u;
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const bool valid = q * 4 + c < end;
+                const uint32_t d = (kk[c] >> shift) & (RADIX - 1);
+                const uint64_t act = __ballot(valid);
+                const int first = act ? __ffsll((unsigned long long)act) - 1 : 0;  // lowest valid lane
+                const uint32_t d0 = (uint32_t)__shfl((int)d, first, 64);
+                const uint64_t same = __ballot(valid && d == d0);
+                if (same == act) {
+                    if (valid && (threadIdx.x & 63) == first) atomicAdd(&s_h[wave][d0], (uint32_t)__popcll(act));
+                } else if (valid) {
+                    atomicAdd(&s_h[wave][d], 1u);
+                }
+            }
+        }
+    } else {
+        for (int64_t q = begin / 4 + tid; q * 4 < end; q += kThreads) {
+            if (q * 4 + 3 < end) {
+                const uint4 v = k4[q];
+                atomicAdd(&s_h[wave][(v.x >> shift) & (RADIX - 1)], 1u);
+                atomicAdd(&s_h[wave][(v.y >> shift) & (RADIX - 1)], 1u);
+                atomicAdd(&s_h[wave][(v.z >> shift) & (RADIX - 1)], 1u);
+                atomicAdd(&s_h[wave][(v.w >> shift) & (RADIX - 1)], 1u);
+            } else {
+                for (int64_t j = q * 4; j < end; ++j) atomicAdd(&s_h[wave][(keys[j] >> shift) & (RADIX - 1)], 1u);
+            }
         }
     }
     __syncthreads();
@@ -313,8 +352,12 @@ template <int BITS>
 hipError_t radix_pass(TableScratch& s, const uint32_t* kin, const int32_t* vin, int id_mode, int32_t N, uint32_t* kout,
                       int32_t* vout, int64_t n, const uint32_t* n_dev, int shift, int nblk, const int32_t* done,
                       hipStream_t st, LaunchTimer* tm) {
-    launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS>, dim3(nblk), dim3(kThreads), st, kin, n, n_dev, shift, s.hist, nblk,
-           done);
+    if (s.grouped_keys)
+        launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, true>, dim3(nblk), dim3(kThreads), st, kin, n, n_dev, shift,
+               s.hist, nblk, done);
+    else
+        launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, false>, dim3(nblk), dim3(kThreads), st, kin, n, n_dev, shift,
+               s.hist, nblk, done);
     hipError_t e = exclusive_scan_u32(s.hist, s.hist, (int64_t)nblk << BITS, s.scan_part, done, st, tm, KC_SORT_SCAN);
     if (e != hipSuccess) return e;
     launch(tm, KC_SORT_DOWNSWEEP, k_downsweep<BITS>, dim3(nblk), dim3(kThreads), st, kin, vin, id_mode, N, kout, vout,
