@@ -13,12 +13,16 @@ namespace pt {
 
 namespace {
 
-constexpr int kThreads = 256;
+#ifndef PT_SORT_THREADS
+#define PT_SORT_THREADS 512
+#endif
+constexpr int kThreads = PT_SORT_THREADS;
+constexpr int kWaves = kThreads / 64;
 #ifndef PT_SORT_ITEMS
-#define PT_SORT_ITEMS 16
+#define PT_SORT_ITEMS 8
 #endif
 constexpr int kItems = PT_SORT_ITEMS;  // pairs per thread per tile
-constexpr int kTile = kThreads * kItems;  // 4096 pairs per tile
+constexpr int kTile = kThreads * kItems;  // pairs per tile
 #ifndef PT_SORT_BLOCKS
 #define PT_SORT_BLOCKS 1024
 #endif
@@ -56,15 +60,15 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
     return v;
 }
 
-// Exclusive scan of one value per thread over a 256-thread block.
-__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_w, uint32_t* total) {
+// Exclusive scan of one value per thread over the block (s_w: kWaves words).
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, uint32_t* total) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t inc = wave_incl_scan(v, lane);
     if (lane == 63) s_w[wave] = inc;
     __syncthreads();
     uint32_t wbase = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < kThreads / 64; ++w) {
+    for (int w = 0; w < kWaves; ++w) {
         const uint32_t x = s_w[w];
         if (w < wave) wbase += x;
         tot += x;
@@ -81,9 +85,9 @@ __global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t* __restrict
                                                       uint32_t* __restrict__ hist, int nblk, const int32_t* done) {
     if (is_done(done)) return;
     constexpr int RADIX = 1 << BITS;
-    __shared__ uint32_t s_h[4][RADIX];
+    __shared__ uint32_t s_h[kWaves][RADIX];
     const int tid = threadIdx.x, wave = tid >> 6;
-    for (int i = tid; i < 4 * RADIX; i += kThreads) (&s_h[0][0])[i] = 0;
+    for (int i = tid; i < kWaves * RADIX; i += kThreads) (&s_h[0][0])[i] = 0;
     __syncthreads();
     int64_t begin, end;
     block_range(n_host, n_dev, begin, end);
@@ -133,14 +137,18 @@ __global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t* __restrict
         }
     }
     __syncthreads();
-    for (int d = tid; d < RADIX; d += kThreads)
-        hist[(int64_t)d * nblk + blockIdx.x] = s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d];
+    for (int d = tid; d < RADIX; d += kThreads) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) t += s_h[w][d];
+        hist[(int64_t)d * nblk + blockIdx.x] = t;
+    }
 }
 
 // Stable scatter of the block's chunk by one digit.  Each tile of 4096 pairs
 // is ranked per wave with ballots (match-any over the digit bits): the lowest
 // lane of each digit group adds the group's size to the wave's LDS counter
-// with a returning atomic, so the 16 rounds of a tile issue back to back and
+// with a returning atomic, so the 8 rounds of a wave issue back to back and
 // LDS order gives each round the count of the rounds before it.  The tile is
 // then staged in LDS in digit order and written out as runs per digit, while
 // the next tile's pairs are already in flight.
@@ -157,11 +165,11 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
     constexpr uint32_t MASK = RADIX - 1;
     __shared__ uint32_t s_keys[kTile];
     __shared__ int32_t s_vals[kTile];
-    __shared__ uint32_t s_wcnt[4][RADIX];
+    __shared__ uint32_t s_wcnt[kWaves][RADIX];
     __shared__ uint32_t s_start[RADIX];
     __shared__ uint32_t s_tot[RADIX];
     __shared__ uint32_t s_base[RADIX];
-    __shared__ uint32_t s_w[4];
+    __shared__ uint32_t s_w[kWaves];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int64_t begin, end;
@@ -181,7 +189,7 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
 
     for (int64_t t0 = begin; t0 < end; t0 += kTile) {
         const int tile_n = (int)min((int64_t)kTile, end - t0);
-        for (int i = tid; i < 4 * RADIX; i += kThreads) (&s_wcnt[0][0])[i] = 0;
+        for (int i = tid; i < kWaves * RADIX; i += kThreads) (&s_wcnt[0][0])[i] = 0;
         __syncthreads();
         // Next tile in flight while this one is ranked and written.
         uint32_t nk[kItems];
@@ -219,12 +227,14 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
         for (int r = 0; r < kItems; ++r) rank[r] = __shfl(old[r], (int)(info[r] >> 8), 64) + (info[r] & 0xFF);
         __syncthreads();
         for (int d = tid; d < RADIX; d += kThreads) {
-            const uint32_t c0 = s_wcnt[0][d], c1 = s_wcnt[1][d], c2 = s_wcnt[2][d], c3 = s_wcnt[3][d];
-            s_wcnt[0][d] = 0;
-            s_wcnt[1][d] = c0;
-            s_wcnt[2][d] = c0 + c1;
-            s_wcnt[3][d] = c0 + c1 + c2;
-            s_tot[d] = c0 + c1 + c2 + c3;
+            uint32_t run = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                const uint32_t c = s_wcnt[w][d];
+                s_wcnt[w][d] = run;
+                run += c;
+            }
+            s_tot[d] = run;
         }
         __syncthreads();
         {
@@ -236,7 +246,7 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
                 loc[j] = d < RADIX ? s_tot[d] : 0u;
                 sum += loc[j];
             }
-            uint32_t run = block_excl_scan256(sum, s_w, nullptr);
+            uint32_t run = block_excl_scan(sum, s_w, nullptr);
 #pragma unroll
             for (int j = 0; j < DPT; ++j) {
                 const int d = tid * DPT + j;
@@ -277,7 +287,7 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
 __global__ __launch_bounds__(kThreads) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
                                                           uint32_t* __restrict__ part, const int32_t* done) {
     if (is_done(done)) return;
-    __shared__ uint32_t s_w[4];
+    __shared__ uint32_t s_w[kWaves];
     const int64_t base = (int64_t)blockIdx.x * kScanSeg;
     uint32_t sum = 0;
     for (int i = threadIdx.x; i < kScanSeg; i += kThreads) {
@@ -285,20 +295,20 @@ __global__ __launch_bounds__(kThreads) void k_scan_reduce(const uint32_t* __rest
         if (j < n) sum += in[j];
     }
     uint32_t tot;
-    block_excl_scan256(sum, s_w, &tot);
+    block_excl_scan(sum, s_w, &tot);
     if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
 // Scan, phase 2: exclusive scan of the segment sums (one block).
 __global__ __launch_bounds__(kThreads) void k_scan_parts(uint32_t* part, int nparts, const int32_t* done) {
     if (is_done(done)) return;
-    __shared__ uint32_t s_w[4];
+    __shared__ uint32_t s_w[kWaves];
     uint32_t carry = 0;
     for (int b0 = 0; b0 < nparts; b0 += kThreads) {
         const int i = b0 + threadIdx.x;
         const uint32_t v = i < nparts ? part[i] : 0u;
         uint32_t tot;
-        const uint32_t ex = block_excl_scan256(v, s_w, &tot);
+        const uint32_t ex = block_excl_scan(v, s_w, &tot);
         if (i < nparts) part[i] = carry + ex;
         carry += tot;
     }
@@ -309,7 +319,7 @@ __global__ __launch_bounds__(kThreads) void k_scan_down(const uint32_t* __restri
                                                         int64_t n, const uint32_t* __restrict__ part,
                                                         const int32_t* done) {
     if (is_done(done)) return;
-    __shared__ uint32_t s_w[4];
+    __shared__ uint32_t s_w[kWaves];
     const int64_t base = (int64_t)blockIdx.x * kScanSeg + (int64_t)threadIdx.x * kScanItems;
     uint32_t x[kScanItems];
     uint32_t sum = 0;
@@ -319,7 +329,7 @@ __global__ __launch_bounds__(kThreads) void k_scan_down(const uint32_t* __restri
         x[i] = j < n ? in[j] : 0u;
         sum += x[i];
     }
-    uint32_t run = block_excl_scan256(sum, s_w, nullptr) + part[blockIdx.x];
+    uint32_t run = block_excl_scan(sum, s_w, nullptr) + part[blockIdx.x];
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
         const int64_t j = base + i;
