@@ -12,7 +12,15 @@ value = person-days per second over the timed steps
 
 roofline: the dominant kernel class by HIP-event time; `achieved` is its
 algorithmic bytes (each logical array read or written once, at its stored
-width) divided by its launch time, both summed over the timed steps.
+width, over the pairs the engine's per-day table log reports) divided by its
+launch time, both summed over the timed steps.  `traffic` is the class's
+measured HBM bytes per launch from the committed rocprofv3 summary
+(profiles/daily_loop_50M_100d.md: FETCH_SIZE + WRITE_SIZE runs of this same
+command, calibrated on the copy kernel).
+
+replicates: the 256-scenario set of pandemicthrust_amd/data/
+config2_scenarios.csv run to the end, setup included, once through the batch
+engine and once one simulation at a time; replicates per second both ways.
 
 cpu_baseline: the CPU oracle (a sequential port of the reference semantics)
 on one host core, timed on a bounded sample of the same workload: setup plus
@@ -20,6 +28,7 @@ the first CPU_DAYS days (setup excluded from the rate).
 """
 import argparse
 import json
+import re
 import os
 import subprocess
 import sys
@@ -97,6 +106,20 @@ def algorithmic_bytes(N, W, lay, inf, counts, full, pairs):
     return per, loop
 
 
+PROFILE = os.path.join(ROOT, "profiles", "daily_loop_50M_100d.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    try:
+        text = open(PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_per_launch %s: (\d+) bytes" % re.escape(kernel), text)
+    return int(m.group(1)) if m else None
+
+
 def cpu_baseline():
     exe = os.path.join(ROOT, "oracle", "build", "pt_oracle")
     if not os.path.exists(exe):
@@ -115,7 +138,7 @@ def cpu_baseline():
                       "(%d agents, %.1f s loop)" % (r["setup_s"], r["days"], r["n_people"], r["loop_s"])}
 
 
-def replicate_bench(n_reps):
+def replicate_bench(n_reps, plain=True):
     """256-scenario ensemble (pandemicthrust_amd/data/config2_scenarios.csv),
     each replicate run to its end (180 days or extinction), setup included:
     plain = one single-simulation handle after another; batched = one batch.
@@ -135,6 +158,15 @@ def replicate_bench(n_reps):
     b.close()
     print("replicates: batched %d in %.3f s (loop %.1f ms)" % (len(rows), batched_s, batched_loop_ms),
           file=sys.stderr, flush=True)
+    person_days = int(sum(int(n_people[r]) * len(res[r][1]) for r in range(len(rows))))
+    out = {"workload": "%d scenarios of pandemicthrust_amd/data/config2_scenarios.csv (N 50k-2M log-uniform, "
+                       "max_days 180), setup + run to the end" % len(rows),
+           "agents": info["agents"], "person_days": person_days,
+           "batched_per_s": round(len(rows) / batched_s, 2), "batched_s": round(batched_s, 3),
+           "batched_loop_ms": round(batched_loop_ms, 1),
+           "batched_person_days_per_s": round(person_days / (batched_loop_ms * 1e-3), 1)}
+    if not plain:
+        return out
     identical = True
     t0 = time.perf_counter()
     for r, (p, s) in enumerate(zip(params, seeds)):
@@ -145,15 +177,9 @@ def replicate_bench(n_reps):
         identical &= bool(np.array_equal(counts, res[r][0]) and np.array_equal(inf, res[r][1]))
     plain_s = time.perf_counter() - t0
     print("replicates: plain %d in %.3f s" % (len(rows), plain_s), file=sys.stderr, flush=True)
-    person_days = int(sum(int(n_people[r]) * len(res[r][1]) for r in range(len(rows))))
-    return {"workload": "%d scenarios of pandemicthrust_amd/data/config2_scenarios.csv (N 50k-2M log-uniform, "
-                        "max_days 180), setup + run to the end" % len(rows),
-            "agents": info["agents"], "person_days": person_days,
-            "batched_per_s": round(len(rows) / batched_s, 2), "plain_per_s": round(len(rows) / plain_s, 2),
-            "speedup": round(plain_s / batched_s, 2), "batched_s": round(batched_s, 3),
-            "batched_loop_ms": round(batched_loop_ms, 1), "plain_s": round(plain_s, 3),
-            "batched_person_days_per_s": round(person_days / (batched_loop_ms * 1e-3), 1),
-            "identical_results": identical}
+    out.update({"plain_per_s": round(len(rows) / plain_s, 2), "plain_s": round(plain_s, 3),
+                "speedup": round(plain_s / batched_s, 2), "identical_results": identical})
+    return out
 
 
 def main():
@@ -163,6 +189,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--replicates", type=int, default=256, help="scenarios in the replicate leg (0 skips it)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched replicate leg (for profiling the batch engine)")
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -170,6 +198,9 @@ def main():
               "(multi-GPU sharding is a design note, not part of this build); use --gpus 1", file=sys.stderr)
         return 2
     if int(os.environ.get("RANK", "0")) != 0:
+        return 0
+    if args.batched_only:
+        print(json.dumps({"replicates": replicate_bench(args.replicates, plain=False)}), flush=True)
         return 0
 
     from pandemicthrust_amd import Params, PandemicSim
@@ -253,7 +284,8 @@ def main():
         "config": {"workload": "50M agents x 100 days: constants,100,1.8,1.3,10,10,21.24,21.24,1,0 seed 1",
                    "n_people": N, "days_per_step": days_per_step, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
+                     "traffic_source": os.path.relpath(PROFILE, ROOT),
                      "avg_launch_us": round(1000 * dom_ms / dom_launches, 2),
                      "algorithmic_bytes_per_launch": round(dom_bytes / dom_launches, 1)},
         "loop_roofline": {"algorithmic_GB_per_loop": round(loop_bytes / 1e9, 3),
