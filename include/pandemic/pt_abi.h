@@ -40,6 +40,23 @@ typedef struct pt_params {
     float asymp_factor;
 } pt_params;
 
+/* Interventions.  The reference has none; the thesis sketches closures and
+ * vaccination as future work (thesis.pdf p.18-19).  The neutral set
+ * {0,0}, {10,10}, 0, 0, 0, 0, {0,0} reproduces the reference exactly.
+ * Contact drops and vaccination draw from their own Threefry stream, so they
+ * consume no draw of the model's stream. */
+typedef struct pt_interventions {
+    int32_t latent_days[2];      /* per strain, 0..30: infectious from this many days after infection */
+    int32_t infectious_days[2];  /* per strain, 1..10: length of the infectious window (shedding
+                                    profile truncated), recovery after latent + infectious days */
+    int32_t school_close_start;  /* first day of a weekday school closure */
+    int32_t school_close_days;   /* closure length (0: schools stay open); closed schools drop
+                                    school and after-school contacts */
+    float workplace_reduction;   /* 0..1: share of adults' workplace contacts dropped */
+    float errand_reduction;      /* 0..1: share of errand contacts dropped */
+    float vaccination[2];        /* 0..1: share of agents immune to each strain from setup */
+} pt_interventions;
+
 typedef struct pt_sim pt_sim;
 
 /* Text of the last error raised on the calling thread. */
@@ -59,6 +76,10 @@ int pt_glibc_initial_infected(int32_t core_seed, int32_t n_people, int32_t n, in
  * setup_pushDeviceData (:569).  seed_key = SEED[0], SEED[1] (the Threefry
  * key).  device = HIP device ordinal. */
 int pt_sim_create(const pt_params* params, const int32_t seed_key[2], int32_t device, pt_sim** out);
+
+/* Sets the interventions of a simulation (no counterpart in the reference;
+ * for scenario sweeps).  Call before pt_sim_setup; default: none. */
+int pt_sim_set_interventions(pt_sim* sim, const pt_interventions* iv);
 
 /* setupSim (PandemicSim.cu:146-208) up to, not including, the initial
  * infections: population size, households, workplaces, workplace table and

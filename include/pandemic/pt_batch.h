@@ -62,6 +62,10 @@ int pt_batch_generation_offsets(pt_batch* b, int32_t r, int32_t* c_pandemic, int
 int64_t pt_batch_array_bytes(pt_batch* b, int32_t r, int32_t which);
 int pt_batch_copy_array(pt_batch* b, int32_t r, int32_t which, void* dst, int64_t bytes);
 
+/* Interventions of every replicate (iv[r], as pt_sim_set_interventions);
+ * call before pt_batch_setup.  Default: none. */
+int pt_batch_set_interventions(pt_batch* b, const pt_interventions* iv);
+
 /* As pt_sim_set_table_mode; applies to every replicate. */
 int pt_batch_set_table_mode(pt_batch* b, int32_t mode);
 

@@ -174,6 +174,9 @@ struct or_sim {
     std::vector<int32_t> hh_offsets, household, wp_offsets, wp_people, wp_keys, infected;
     std::vector<int32_t> errand_offsets, errand_people;
     int errand_hours_today = 0;
+    /* interventions */
+    or_interventions iv{{0, 0}, {CULMINATION, CULMINATION}, 0, 0, 0.f, 0.f, {0.f, 0.f}};
+    uint64_t ik0 = 0, ik1 = 0;
     /* trace */
     bool trace_on = false;
     std::vector<int32_t> tr_infector, tr_victim;
@@ -489,15 +492,15 @@ void process_contacts(or_sim* s, int32_t idx, int kval_sum, const int32_t* vic, 
     float prob_p = -1.f, prob_s = -1.f;
     uint8_t gp = GEN_NONE, gs = GEN_NONE;
     if (sp >= ST_INFECTED) {
-        const int pd = d - (int)s->day_p[idx];
-        if (pd >= 0 && pd < CULMINATION) {
+        const int pd = d - (int)s->day_p[idx] - s->iv.latent_days[0];
+        if (pd >= 0 && pd < s->iv.infectious_days[0]) {
             prob_p = infection_probability(s, sp - ST_INFECTED, pd, 0, kval_sum);
             gp = (uint8_t)(s->gen_p[idx] + 1);
         }
     }
     if (ss >= ST_INFECTED) {
-        const int pd = d - (int)s->day_s[idx];
-        if (pd >= 0 && pd < CULMINATION) {
+        const int pd = d - (int)s->day_s[idx] - s->iv.latent_days[1];
+        if (pd >= 0 && pd < s->iv.infectious_days[1]) {
             prob_s = infection_probability(s, ss - ST_INFECTED, pd, 1, kval_sum);
             gs = (uint8_t)(s->gen_s[idx] + 1);
         }
@@ -560,15 +563,16 @@ void bucket_stable(const std::vector<int32_t>& keys, const std::vector<int32_t>&
  * (PandemicSim.cu:2738-2768, :2097-2192). */
 void count_and_recover(or_sim* s, int32_t counts[16]) {
     for (int i = 0; i < 16; ++i) counts[i] = 0;
-    const int day_to_recover = s->current_day - CULMINATION;
+    const int rec_p = s->current_day - (s->iv.latent_days[0] + s->iv.infectious_days[0]);
+    const int rec_s = s->current_day - (s->iv.latent_days[1] + s->iv.infectious_days[1]);
     for (int i = 0; i < s->N; ++i) {
         int sp = s->st_p[i];
-        if (day_to_recover >= 0 && sp >= ST_INFECTED && (int)s->day_p[i] <= day_to_recover) {
+        if (rec_p >= 0 && sp >= ST_INFECTED && (int)s->day_p[i] <= rec_p) {
             sp = ST_RECOVERED;
             s->st_p[i] = ST_RECOVERED;
         }
         int ss = s->st_s[i];
-        if (day_to_recover >= 0 && ss >= ST_INFECTED && (int)s->day_s[i] <= day_to_recover) {
+        if (rec_s >= 0 && ss >= ST_INFECTED && (int)s->day_s[i] <= rec_s) {
             ss = ST_RECOVERED;
             s->st_s[i] = ST_RECOVERED;
         }
@@ -626,6 +630,30 @@ void build_weekend_tables(or_sim* s) {
     s->errand_hours_today = WE_HOURS;
 }
 
+/* Contact interventions (or_interventions) for one infector's chosen
+ * contacts; the normalising contact count is not changed. */
+void drop_contacts(const or_sim* s, bool weekend, uint64_t M, int C, int32_t* vic, uint8_t* typ) {
+    const or_interventions& iv = s->iv;
+    if (iv.school_close_days == 0 && iv.workplace_reduction == 0.f && iv.errand_reduction == 0.f) return;
+    const Words4 x = draw4(M, s->ik0, s->ik1);
+    const bool closed = !weekend && s->current_day >= iv.school_close_start &&
+                        s->current_day < iv.school_close_start + iv.school_close_days;
+    for (int c = 0; c < C; ++c) {
+        const float u16 = (float)((x.w[c / 2] >> (16 * (c % 2))) & 0xFFFFu) * (1.0f / 65536.0f);
+        bool drop = false;
+        if (typ[c] == CT_SCHOOL || typ[c] == CT_AFTERSCHOOL)
+            drop = closed;
+        else if (typ[c] == CT_WORKPLACE)
+            drop = u16 < iv.workplace_reduction;
+        else if (typ[c] == CT_ERRAND)
+            drop = u16 < iv.errand_reduction;
+        if (drop) {
+            vic[c] = -1;
+            typ[c] = CT_NONE;
+        }
+    }
+}
+
 void contacts_day(or_sim* s, bool weekend) {
     const int I = (int)s->infected.size();
     const int C = weekend ? C_WE : C_WD;
@@ -645,6 +673,7 @@ void contacts_day(or_sim* s, bool weekend) {
             kval = make_contacts_weekend(s, idx, M, vic, typ);
         else
             kval = make_contacts_weekday(s, idx, s->age[idx], M, vic, typ);
+        drop_contacts(s, weekend, M, C, vic, typ);
         process_contacts(s, idx, kval, vic, typ, C, M + 2, s->trace_on ? (int64_t)pos * C : -1);
     }
     s->rand_offset += (uint64_t)(6 * (int64_t)I);
@@ -701,11 +730,15 @@ or_sim* or_create(const or_params* p, int32_t core_seed) {
     or_glibc_seeds(core_seed, s->seed);
     s->k0 = (uint64_t)(int64_t)s->seed[0];
     s->k1 = (uint64_t)(int64_t)s->seed[1];
+    s->ik0 = s->k0 ^ 0x9E3779B97F4A7C15ull; /* intervention stream key */
+    s->ik1 = s->k1 ^ 0xBF58476D1CE4E5B9ull;
     scale_and_calibrate(s);
     return s;
 }
 
 void or_destroy(or_sim* s) { delete s; }
+
+void or_set_interventions(or_sim* s, const or_interventions* iv) { s->iv = *iv; }
 
 /* setupSim (PandemicSim.cu:146-208) */
 int or_setup(or_sim* s) {
@@ -791,6 +824,12 @@ int or_setup(or_sim* s) {
     s->day_s.assign((size_t)N, DAY_NONE);
     s->gen_p.assign((size_t)N, GEN_NONE);
     s->gen_s.assign((size_t)N, GEN_NONE);
+    if (s->iv.vaccination[0] > 0.f || s->iv.vaccination[1] > 0.f)
+        for (int i = 0; i < N; ++i) {
+            const Words4 x = draw4((1ull << 63) | (uint64_t)i, s->ik0, s->ik1);
+            if (unit_float(x.w[0]) < s->iv.vaccination[0]) s->st_p[i] = ST_RECOVERED;
+            if (unit_float(x.w[1]) < s->iv.vaccination[1]) s->st_s[i] = ST_RECOVERED;
+        }
     const int ninit = s->p.init_pandemic + s->p.init_seasonal;
     std::vector<int32_t> init((size_t)ninit);
     if (or_glibc_initial_infected(s->core_seed, N, ninit, init.data()) != 0) return -1;

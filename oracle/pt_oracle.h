@@ -33,6 +33,29 @@ typedef struct or_params {
     float asymp_factor;
 } or_params;
 
+/* Interventions (not in the reference; a build extension).  Neutral:
+ * latent {0,0}, infectious {10,10}, no closure, no reductions, no
+ * vaccination — the reference exactly.  Infectious while latent <= day -
+ * infection day < latent + infectious (shedding index day - infection day -
+ * latent), recovery at latent + infectious days.  On weekdays inside
+ * [school_close_start, +school_close_days) school and after-school contacts
+ * are dropped; a workplace (errand) contact in slot c is dropped when
+ * uniform16(c) < workplace (errand) reduction, uniform16(c) being 16-bit
+ * field c of Threefry2x64(ctr = the infector's contact counter, key = SEED
+ * XOR (0x9E3779B97F4A7C15, 0xBF58476D1CE4E5B9)); dropped slots become
+ * (-1, NONE) and the infection probability keeps the count of contacts chosen
+ * before the drops.  Agent i is immune to strain k from setup when word k of
+ * the same stream at counter 2^63 | i, over 2^32, is below vaccination[k]. */
+typedef struct or_interventions {
+    int32_t latent_days[2];
+    int32_t infectious_days[2];
+    int32_t school_close_start;
+    int32_t school_close_days;
+    float workplace_reduction;
+    float errand_reduction;
+    float vaccination[2];
+} or_interventions;
+
 typedef struct or_sim or_sim;
 
 /* Array ids for or_get_array. */
@@ -67,6 +90,8 @@ const char* or_last_error(void);
 
 /* Setup (K1-K7).  Returns 0 on success. */
 int or_setup(or_sim* s);
+/* Interventions for the next or_setup (default: none). */
+void or_set_interventions(or_sim* s, const or_interventions* iv);
 
 /* One reference loop iteration.  Writes the day's 16 status counts and I.
  * Returns 0 if the day ran contacts, 1 if it was the terminal I==0 day or

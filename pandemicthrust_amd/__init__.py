@@ -5,4 +5,4 @@ include/pandemic/pt_abi.h, plus the C++ host program (host/) that keeps the
 reference's PandemicSim flow and file contract.  This package exposes a
 ctypes binding of the same ABI for Python callers.
 """
-from .api import PTError, Params, PandemicBatch, PandemicSim, glibc_initial_infected, glibc_seeds, lib  # noqa: F401
+from .api import Interventions, PTError, Params, PandemicBatch, PandemicSim, glibc_initial_infected, glibc_seeds, lib  # noqa: F401

@@ -24,7 +24,7 @@ EXPORTS = [
     "pt_sim_seed_infections", "pt_sim_step", "pt_sim_run", "pt_sim_generation_offsets", "pt_sim_array_bytes",
     "pt_sim_copy_array", "pt_sim_get_scalars", "pt_sim_get_layout", "pt_sim_set_table_mode", "pt_sim_table_log",
     "pt_sim_snapshot",
-    "pt_sim_restore", "pt_sim_enable_timing",
+    "pt_sim_restore", "pt_sim_enable_timing", "pt_sim_set_interventions", "pt_batch_set_interventions",
     "pt_sim_kernel_times", "pt_sim_kernel_name", "pt_sim_synchronize", "pt_sim_last_run_ms", "pt_sim_destroy",
     "pt_batch_create", "pt_batch_setup", "pt_batch_seed_infections", "pt_batch_get_info", "pt_batch_run",
     "pt_batch_generation_offsets", "pt_batch_array_bytes", "pt_batch_copy_array", "pt_batch_set_table_mode",
@@ -51,6 +51,25 @@ class Params(C.Structure):
         f = [x.strip() for x in fields]
         return cls(int(f[0]), float(f[1]), float(f[2]), int(f[3]), int(f[4]), float(f[5]), float(f[6]),
                    float(f[7]), float(f[8]))
+
+
+class Interventions(C.Structure):
+    """pt_interventions (include/pandemic/pt_abi.h); the defaults are the
+    neutral set, which reproduces the reference exactly."""
+    _fields_ = [
+        ("latent_days", C.c_int32 * 2),
+        ("infectious_days", C.c_int32 * 2),
+        ("school_close_start", C.c_int32),
+        ("school_close_days", C.c_int32),
+        ("workplace_reduction", C.c_float),
+        ("errand_reduction", C.c_float),
+        ("vaccination", C.c_float * 2),
+    ]
+
+    def __init__(self, latent=(0, 0), infectious=(10, 10), school_close_start=0, school_close_days=0,
+                 workplace_reduction=0.0, errand_reduction=0.0, vaccination=(0.0, 0.0)):
+        super().__init__((C.c_int32 * 2)(*latent), (C.c_int32 * 2)(*infectious), school_close_start,
+                         school_close_days, workplace_reduction, errand_reduction, (C.c_float * 2)(*vaccination))
 
 
 class PTError(RuntimeError):
@@ -82,6 +101,8 @@ def lib():
         L.pt_sim_get_scalars.argtypes = [C.c_void_p, C.c_void_p]
         L.pt_sim_get_layout.argtypes = [C.c_void_p, C.c_void_p]
         L.pt_sim_set_table_mode.argtypes = [C.c_void_p, C.c_int32]
+        L.pt_sim_set_interventions.argtypes = [C.c_void_p, C.POINTER(Interventions)]
+        L.pt_batch_set_interventions.argtypes = [C.c_void_p, C.c_void_p]
         L.pt_sim_table_log.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
         L.pt_sim_snapshot.argtypes = [C.c_void_p]
         L.pt_sim_restore.argtypes = [C.c_void_p]
@@ -210,6 +231,10 @@ class PandemicSim:
         """0 automatic, 1 full errand tables every day, 2 queried buckets only."""
         _check(lib().pt_sim_set_table_mode(self.h, mode))
 
+    def set_interventions(self, iv):
+        """Interventions (an Interventions); call before setup_sim()."""
+        _check(lib().pt_sim_set_interventions(self.h, C.byref(iv)))
+
     def table_log(self):
         """Per day since setup / restore: (full table?, pairs sorted)."""
         full = np.zeros(256, np.int32)
@@ -329,6 +354,11 @@ class PandemicBatch:
 
     def set_table_mode(self, mode):
         _check(lib().pt_batch_set_table_mode(self.h, mode))
+
+    def set_interventions(self, ivs):
+        """One Interventions per replicate; call before setup()."""
+        arr = (Interventions * self.n)(*ivs)
+        _check(lib().pt_batch_set_interventions(self.h, arr))
 
     def snapshot(self):
         _check(lib().pt_batch_snapshot(self.h))

@@ -113,7 +113,7 @@ constexpr int kRepThreads = 256;
 
 // D1 + D2 of every running replicate.
 __global__ __launch_bounds__(kSweepThreads) void kb_sweep(const DayState* __restrict__ g, const DayState* __restrict__ sts,
-                                                          const RepInfo* __restrict__ reps,
+                                                          const RepInfo* __restrict__ reps, const Model* __restrict__ models,
                                                           const int32_t* __restrict__ tile_rep, uint8_t* __restrict__ st_p,
                                                           uint8_t* __restrict__ st_s, const uint8_t* __restrict__ dy_p,
                                                           const uint8_t* __restrict__ dy_s,
@@ -125,7 +125,8 @@ __global__ __launch_bounds__(kSweepThreads) void kb_sweep(const DayState* __rest
     if (sts[r].done) return;
     const RepInfo ri = reps[r];
     const int64_t a = ri.abase;
-    sweep_block(ri.N, sts[r].day, (b - ri.tile0) * (int64_t)kSweepTile, st_p + a, st_s + a, dy_p + a, dy_s + a,
+    sweep_block(models[r], ri.N, sts[r].day, (b - ri.tile0) * (int64_t)kSweepTile, st_p + a, st_s + a, dy_p + a,
+                dy_s + a,
                 blk_counts + b * 16, blk_ninf + b, staging + b * kSweepTile, (int32_t)a);
 }
 
@@ -589,7 +590,7 @@ int enqueue_batch_day(pt_batch* b, int d) {
     hipStream_t s = b->stream;
     DayState* g = b->d_g;
     const int R = b->R;
-    kb_sweep<<<b->ntiles, kSweepThreads, 0, s>>>(g, b->d_states, b->d_reps, b->tile_rep, b->st_p, b->st_s, b->dy_p,
+    kb_sweep<<<b->ntiles, kSweepThreads, 0, s>>>(g, b->d_states, b->d_reps, b->d_models, b->tile_rep, b->st_p, b->st_s, b->dy_p,
                                                  b->dy_s, b->blk_counts, b->blk_ninf, b->staging);
     kb_plan<<<R, kRepThreads, 0, s>>>(g, b->d_states, b->d_reps, b->d_models, b->blk_counts, b->blk_ninf, b->blk_off,
                                       b->rep_I, b->d_counts, b->d_inf, b->D, b->table_mode);
@@ -954,6 +955,18 @@ int pt_batch_copy_array(pt_batch* b, int32_t r, int32_t which, void* dst, int64_
         case PT_HH_TYPES: src = b->hh_types + ri.hbase; break;
     }
     HIP_TRY(hipMemcpy(dst, src, (size_t)need, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+int pt_batch_set_interventions(pt_batch* b, const pt_interventions* iv) {
+    if (!b || !iv) return fail(PT_EINVAL, "null argument");
+    if (b->setup_done) return fail(PT_ESTATE, "interventions must be set before setup");
+    std::vector<Model> m = b->models;
+    for (int r = 0; r < b->R; ++r) {
+        const int rc = apply_interventions(&iv[r], m[r]);
+        if (rc != PT_OK) return fail(rc, "replicate %d: %s", r, pt_last_error());
+    }
+    b->models = m;
     return PT_OK;
 }
 

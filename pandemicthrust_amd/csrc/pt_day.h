@@ -57,6 +57,10 @@ inline int grid_for(int64_t n, int threads, int cap) {
 // and the dense errand-table layout.  Returns PT_OK or PT_EINVAL.
 int build_model(const pt_params* p, const int32_t seed_key[2], Model& m);
 
+// Validates an intervention set and writes it into the model (PT_OK or
+// PT_EINVAL).
+int apply_interventions(const pt_interventions* iv, Model& m);
+
 // Population arrays of one simulation (local numbering).
 struct PopArrays {
     uint8_t* hh_types;   // H + 1
@@ -155,17 +159,19 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, uint32_t n_mi
 // ---------------- D1 + D2: sweep ----------------
 
 // One sweep block (kSweepThreads threads) over agents [t0, t0 + kSweepTile)
-// of a population of N: recover (day <= current_day - 10), count the 8
-// status codes per strain and stage the ids (+ id_bias) of infected agents.
+// of a population of N: recover (infection day <= current_day - (latent +
+// infectious days), the reference's current_day - 10), count the 8 status
+// codes per strain and stage the ids (+ id_bias) of infected agents.
 // (kernel_countInfectedStatusAndRecover PandemicSim.cu:2097-2192,
 //  daily_buildInfectedArray_global :2207-2224.)  The status arrays hold
 // whole uint4 words past N.
-__device__ __forceinline__ void sweep_block(int64_t N, int day, int64_t t0, uint8_t* __restrict__ st_p,
+__device__ __forceinline__ void sweep_block(const Model& M, int64_t N, int day, int64_t t0, uint8_t* __restrict__ st_p,
                                             uint8_t* __restrict__ st_s, const uint8_t* __restrict__ dy_p,
                                             const uint8_t* __restrict__ dy_s, uint32_t* __restrict__ counts16,
                                             uint32_t* __restrict__ ninf_out, int32_t* __restrict__ stage,
                                             int32_t id_bias) {
-    const int dtr = day - kCulmination;
+    const int dtr_p = day - (M.lat[0] + M.dur[0]);
+    const int dtr_s = day - (M.lat[1] + M.dur[1]);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t a0 = t0 + (int64_t)tid * kSweepPer;
     uint8_t sp[kSweepPer], ss[kSweepPer];
@@ -182,14 +188,14 @@ __device__ __forceinline__ void sweep_block(int64_t N, int day, int64_t t0, uint
             ss[i] = (a0 + i < N) ? st_s[a0 + i] : kSusceptible;
         }
     }
-    if (dtr >= 0) {
+    if (dtr_p >= 0 || dtr_s >= 0) {
         bool anyp = false, anys = false;
 #pragma unroll
         for (int i = 0; i < kSweepPer; ++i) {
             anyp |= sp[i] >= kInfected;
             anys |= ss[i] >= kInfected;
         }
-        if (anyp) {
+        if (anyp && dtr_p >= 0) {
             uint8_t dp[kSweepPer];
             if (full) {
                 const uint4 v = *reinterpret_cast<const uint4*>(dy_p + a0);
@@ -201,7 +207,7 @@ __device__ __forceinline__ void sweep_block(int64_t N, int day, int64_t t0, uint
             bool dirty = false;
 #pragma unroll
             for (int i = 0; i < kSweepPer; ++i)
-                if (sp[i] >= kInfected && (int)dp[i] <= dtr) {
+                if (sp[i] >= kInfected && (int)dp[i] <= dtr_p) {
                     sp[i] = kRecovered;
                     dirty = true;
                 }
@@ -216,7 +222,7 @@ __device__ __forceinline__ void sweep_block(int64_t N, int day, int64_t t0, uint
                 }
             }
         }
-        if (anys) {
+        if (anys && dtr_s >= 0) {
             uint8_t ds[kSweepPer];
             if (full) {
                 const uint4 v = *reinterpret_cast<const uint4*>(dy_s + a0);
@@ -228,7 +234,7 @@ __device__ __forceinline__ void sweep_block(int64_t N, int day, int64_t t0, uint
             bool dirty = false;
 #pragma unroll
             for (int i = 0; i < kSweepPer; ++i)
-                if (ss[i] >= kInfected && (int)ds[i] <= dtr) {
+                if (ss[i] >= kInfected && (int)ds[i] <= dtr_s) {
                     ss[i] = kRecovered;
                     dirty = true;
                 }
@@ -437,29 +443,31 @@ struct AgentArrays {
 };
 
 // Strain state of an infector (device_processContacts, PandemicSim.cu:3125-3175):
-// profile and shedding day if infectious today, else -1.
+// profile and shedding day if infectious today, else -1.  Infectious while
+// lat <= day - infection day < lat + dur (the reference: 0 <= . < 10), at
+// shedding index day - infection day - lat.
 struct Infectivity {
     int prof_p, dd_p, prof_s, dd_s;
     uint8_t gen_p, gen_s;
 };
 
-__device__ __forceinline__ Infectivity infectivity(const AgentArrays& A, int32_t idx, int day) {
+__device__ __forceinline__ Infectivity infectivity(const Model& M, const AgentArrays& A, int32_t idx, int day) {
     Infectivity f;
     f.prof_p = f.prof_s = -1;
     f.dd_p = f.dd_s = 0;
     f.gen_p = f.gen_s = kGenNone;
     const uint8_t sp = A.st_p[idx], ss = A.st_s[idx];
     if (sp >= kInfected) {
-        const int dd = day - (int)A.dy_p[idx];
-        if (dd >= 0 && dd < kCulmination) {
+        const int dd = day - (int)A.dy_p[idx] - M.lat[0];
+        if (dd >= 0 && dd < M.dur[0]) {
             f.prof_p = sp - kInfected;
             f.dd_p = dd;
             f.gen_p = (uint8_t)(A.gn_p[idx] + 1);
         }
     }
     if (ss >= kInfected) {
-        const int dd = day - (int)A.dy_s[idx];
-        if (dd >= 0 && dd < kCulmination) {
+        const int dd = day - (int)A.dy_s[idx] - M.lat[1];
+        if (dd >= 0 && dd < M.dur[1]) {
             f.prof_s = ss - kInfected;
             f.dd_s = dd;
             f.gen_s = (uint8_t)(A.gn_s[idx] + 1);
@@ -554,7 +562,7 @@ __device__ __forceinline__ void weekday_contacts(const Model& M, const AgentArra
                                                  uint64_t eoff, uint64_t workplace_off, int32_t idx, int64_t pos,
                                                  int32_t (&vic)[8], int (&typ)[8], bool (&newp)[8],
                                                  bool (&news)[8]) {
-    const Infectivity f = infectivity(A, idx, day);
+    const Infectivity f = infectivity(M, A, idx, day);
     if (!can_infect(M, f) && !A.trace_victim) return;
     const uint64_t Mc = cbase + (uint64_t)(6 * pos);
     const U4 ua = draw4(Mc, M.key0, M.key1);
@@ -608,6 +616,7 @@ __device__ __forceinline__ void weekday_contacts(const Model& M, const AgentArra
         typ[c] = t;
         kval += t != CT_NONE;
     }
+    apply_drops(M, day, false, Mc, 8, vic, typ);
     if (A.trace_victim) {
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
@@ -623,7 +632,7 @@ __device__ __forceinline__ void weekday_contacts(const Model& M, const AgentArra
 __device__ __forceinline__ void weekend_contacts(const Model& M, const AgentArrays& A, int day, uint64_t cbase,
                                                  uint64_t eoff, int32_t idx, int64_t pos, int32_t (&vic)[8],
                                                  int (&typ)[8], bool (&newp)[8], bool (&news)[8]) {
-    const Infectivity f = infectivity(A, idx, day);
+    const Infectivity f = infectivity(M, A, idx, day);
     if (!can_infect(M, f) && !A.trace_victim) return;
     const uint64_t Mc = cbase + (uint64_t)(6 * pos);
     const U4 u = draw4(Mc, M.key0, M.key1);
@@ -650,6 +659,7 @@ __device__ __forceinline__ void weekend_contacts(const Model& M, const AgentArra
     int kval = 0;
 #pragma unroll
     for (int c = 0; c < 5; ++c) kval += typ[c] != CT_NONE;
+    apply_drops(M, day, true, Mc, 5, vic, typ);
     if (A.trace_victim) {
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
@@ -677,7 +687,7 @@ __device__ __forceinline__ void set_mark(uint32_t* __restrict__ mark, int32_t ke
 // gets one; PandemicSim.cu:1776-1813).
 __device__ __forceinline__ void mark_weekday(const Model& M, const AgentArrays& A, int day, uint64_t eoff, int32_t idx,
                                              uint32_t* __restrict__ mark, uint32_t kbias) {
-    const Infectivity f = infectivity(A, idx, day);
+    const Infectivity f = infectivity(M, A, idx, day);
     if (!can_infect(M, f)) return;
     const bool adult = A.age[idx] == kAgeAdult;
     const U4 ue = draw4(eoff + (uint64_t)(idx / 2), M.key0, M.key1);
@@ -695,7 +705,7 @@ __device__ __forceinline__ void mark_weekday(const Model& M, const AgentArrays& 
 __device__ __forceinline__ void mark_weekend(const Model& M, const AgentArrays& A, int day, uint64_t cbase,
                                              uint64_t eoff, int64_t pos, int32_t idx, uint32_t* __restrict__ mark,
                                              uint32_t kbias) {
-    const Infectivity f = infectivity(A, idx, day);
+    const Infectivity f = infectivity(M, A, idx, day);
     if (!can_infect(M, f)) return;
     const U4 u = draw4(cbase + (uint64_t)(6 * pos), M.key0, M.key1);
     const int prof = (int)(u.w[3] % 6u);

@@ -54,6 +54,19 @@ struct Model {
     int32_t we_base, we_E;
     int32_t wd_base1, wd_E1, wd_K;
     int32_t pad1;
+    // Interventions (none in the reference; the neutral values reproduce it
+    // exactly): per strain, infectious while lat <= day - infection day <
+    // lat + dur (reference: 0, 10); weekday school closure [school_start,
+    // school_start + school_days); shares of workplace / errand contacts
+    // dropped; share of agents immune to each strain from setup.  Drop and
+    // vaccination draws come from their own Threefry stream (key ikey0,
+    // ikey1), so no draw of the model's stream is consumed or moved.
+    int32_t lat[2], dur[2];
+    int32_t school_start, school_days;
+    float wp_red, er_red, vacc[2];
+    int32_t drops;  // any contact can be dropped (school closure or a reduction > 0)
+    int32_t pad2;
+    uint64_t ikey0, ikey1;
 };
 
 // ---------------- Threefry (Salmon et al., SC'11) ----------------
@@ -206,6 +219,48 @@ __device__ __forceinline__ void weekday_dests(const Model& M, uint32_t r1, uint3
     d1 = fish_weekday_dest(M, r1, adult);
     d2 = adult ? fish_weekday_dest(M, r2, adult) : d1;
 }
+
+// ---------------- interventions ----------------
+
+// Sixteen-bit uniform of contact slot c (0..7) from one draw of the
+// intervention stream.
+__host__ __device__ __forceinline__ float drop_uniform(const U4& x, int c) {
+    return (float)((x.w[c >> 1] >> (16 * (c & 1))) & 0xFFFFu) * (1.0f / 65536.0f);
+}
+
+// Contact interventions for one infector, applied after its contacts are
+// chosen: on a weekday inside the closure window school and after-school
+// contacts are dropped; a workplace (errand) contact is dropped when its
+// slot's uniform is below the workplace (errand) reduction.  A dropped slot
+// becomes (victim -1, CT_NONE); the infector's normalising contact count is
+// left as it was, so closing a place removes its transmission rather than
+// moving it to the remaining contacts.  Mc is the infector's contact counter.
+__device__ __forceinline__ void apply_drops(const Model& M, int day, bool weekend, uint64_t Mc, int C,
+                                            int32_t (&vic)[8], int (&typ)[8]) {
+    if (!M.drops) return;
+    const U4 x = draw4(Mc, M.ikey0, M.ikey1);
+    const bool closed = !weekend && day >= M.school_start && day < M.school_start + M.school_days;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        if (c >= C) break;
+        const int t = typ[c];
+        bool drop = false;
+        if (t == CT_SCHOOL || t == CT_AFTERSCHOOL)
+            drop = closed;
+        else if (t == CT_WORKPLACE)
+            drop = drop_uniform(x, c) < M.wp_red;
+        else if (t == CT_ERRAND)
+            drop = drop_uniform(x, c) < M.er_red;
+        if (drop) {
+            vic[c] = -1;
+            typ[c] = CT_NONE;
+        }
+    }
+}
+
+// Counter of agent i's vaccination draw in the intervention stream (kept
+// apart from the contact counters by the top bit).
+__host__ __device__ __forceinline__ uint64_t vaccination_counter(int64_t i) { return (1ull << 63) | (uint64_t)i; }
 
 // Weekday errand-table keys of one person.  Hour 0 spans every destination
 // in [wd_base, W): key d1 - wd_base.  In hour 1 a child stays at its

@@ -149,13 +149,23 @@ __global__ void k_assign_workplaces(const Model* __restrict__ Mp, uint64_t workp
     }
 }
 
-// K7: setup_initializeStatusArrays (PandemicSim.cu:3654-3664).
-__global__ void k_init_status(int64_t N, uint8_t* st_p, uint8_t* st_s, uint8_t* dy_p, uint8_t* dy_s, uint8_t* gn_p,
-                              uint8_t* gn_s) {
+// K7: setup_initializeStatusArrays (PandemicSim.cu:3654-3664).  Agents
+// vaccinated against a strain (intervention stream draw below the coverage;
+// coverage 0 in the reference) start recovered for it.
+__global__ void k_init_status(const Model* __restrict__ Mp, int64_t N, uint8_t* st_p, uint8_t* st_s, uint8_t* dy_p,
+                              uint8_t* dy_s, uint8_t* gn_p, uint8_t* gn_s) {
+    const Model& M = *Mp;
+    const bool vacc = M.vacc[0] > 0.f || M.vacc[1] > 0.f;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
-        st_p[i] = kSusceptible;
-        st_s[i] = kSusceptible;
+        uint8_t sp = kSusceptible, ss = kSusceptible;
+        if (vacc) {
+            const U4 x = draw4(vaccination_counter(i), M.ikey0, M.ikey1);
+            if (unit_float(x.w[0]) < M.vacc[0]) sp = kRecovered;
+            if (unit_float(x.w[1]) < M.vacc[1]) ss = kRecovered;
+        }
+        st_p[i] = sp;
+        st_s[i] = ss;
         dy_p[i] = kDayNone;
         dy_s[i] = kDayNone;
         gn_p[i] = kGenNone;
@@ -171,7 +181,8 @@ __global__ void k_fill_u64(uint64_t* p, int64_t n, uint64_t v) {
 // ============================ daily kernels ============================
 
 // D1 + D2 fused over the whole population (sweep_block).
-__global__ __launch_bounds__(kSweepThreads) void k_sweep(int64_t N, DayState* __restrict__ st,
+__global__ __launch_bounds__(kSweepThreads) void k_sweep(const Model* __restrict__ Mp, int64_t N,
+                                                         DayState* __restrict__ st,
                                                          uint8_t* __restrict__ st_p, uint8_t* __restrict__ st_s,
                                                          const uint8_t* __restrict__ dy_p,
                                                          const uint8_t* __restrict__ dy_s,
@@ -180,7 +191,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(int64_t N, DayState* __
                                                          int32_t* __restrict__ staging) {
     if (st->done) return;
     const int64_t b = blockIdx.x;
-    sweep_block(N, st->day, b * kSweepTile, st_p, st_s, dy_p, dy_s, blk_counts + b * 16, blk_ninf + b,
+    sweep_block(*Mp, N, st->day, b * kSweepTile, st_p, st_s, dy_p, dy_s, blk_counts + b * 16, blk_ninf + b,
                 staging + b * kSweepTile, 0);
 }
 
@@ -624,6 +635,39 @@ int build_model(const pt_params* p, const int32_t seed_key[2], Model& m) {
     m.wd_base1 = std::max(b1, m.wd_base);
     m.wd_E1 = m.W - m.wd_base1;
     m.wd_K = m.wd_E + m.wd_E1 + 1;
+    // No interventions: the reference's 10-day infectious window from the
+    // day of infection; the intervention stream gets its own key.
+    m.dur[0] = m.dur[1] = kCulmination;
+    m.ikey0 = m.key0 ^ 0x9E3779B97F4A7C15ull;
+    m.ikey1 = m.key1 ^ 0xBF58476D1CE4E5B9ull;
+    return PT_OK;
+}
+
+int apply_interventions(const pt_interventions* iv, Model& m) {
+    for (int k = 0; k < 2; ++k) {
+        if (iv->latent_days[k] < 0 || iv->latent_days[k] > 30)
+            return fail(PT_EINVAL, "latent_days[%d] = %d outside [0, 30]", k, iv->latent_days[k]);
+        if (iv->infectious_days[k] < 1 || iv->infectious_days[k] > kCulmination)
+            return fail(PT_EINVAL, "infectious_days[%d] = %d outside [1, %d]", k, iv->infectious_days[k], kCulmination);
+        if (!(iv->vaccination[k] >= 0.f && iv->vaccination[k] <= 1.f))
+            return fail(PT_EINVAL, "vaccination[%d] outside [0, 1]", k);
+    }
+    if (iv->school_close_start < 0 || iv->school_close_days < 0 || iv->school_close_start > 255 ||
+        iv->school_close_days > 255)
+        return fail(PT_EINVAL, "school closure window outside [0, 255]");
+    if (!(iv->workplace_reduction >= 0.f && iv->workplace_reduction <= 1.f) ||
+        !(iv->errand_reduction >= 0.f && iv->errand_reduction <= 1.f))
+        return fail(PT_EINVAL, "contact reductions must lie in [0, 1]");
+    for (int k = 0; k < 2; ++k) {
+        m.lat[k] = iv->latent_days[k];
+        m.dur[k] = iv->infectious_days[k];
+        m.vacc[k] = iv->vaccination[k];
+    }
+    m.school_start = iv->school_close_start;
+    m.school_days = iv->school_close_days;
+    m.wp_red = iv->workplace_reduction;
+    m.er_red = iv->errand_reduction;
+    m.drops = (m.school_days > 0 || m.wp_red > 0.f || m.er_red > 0.f) ? 1 : 0;
     return PT_OK;
 }
 
@@ -646,7 +690,8 @@ hipError_t build_population(const Model* d_model, int64_t N, int H, int W, const
                     nullptr);
     if (e != hipSuccess) return e;
     // K7: statuses; claim words start empty.
-    k_init_status<<<grid_for(N, 256, 65535), 256, 0, st>>>(N, P.st_p, P.st_s, P.dy_p, P.dy_s, P.gn_p, P.gn_s);
+    k_init_status<<<grid_for(N, 256, 65535), 256, 0, st>>>(d_model, N, P.st_p, P.st_s, P.dy_p, P.dy_s, P.gn_p,
+                                                           P.gn_s);
     k_fill_u64<<<grid_for(N, 256, 65535), 256, 0, st>>>(P.claim_p, N, ~0ull);
     k_fill_u64<<<grid_for(N, 256, 65535), 256, 0, st>>>(P.claim_s, N, ~0ull);
     return hipGetLastError();
@@ -770,7 +815,8 @@ int enqueue_day(pt_sim* s, int d) {
     LaunchTimer* tm = &s->timer;
     const int N = (int)s->N;
     DayState* st = s->d_state;
-    launch(tm, KC_SWEEP, k_sweep, dim3(s->nblk_sweep), dim3(kSweepThreads), stc, s->N, st, s->st_p, s->st_s,
+    launch(tm, KC_SWEEP, k_sweep, dim3(s->nblk_sweep), dim3(kSweepThreads), stc, (const Model*)s->d_model, s->N, st,
+           s->st_p, s->st_s,
            (const uint8_t*)s->dy_p, (const uint8_t*)s->dy_s, s->blk_counts, s->blk_ninf, s->staging);
     // Marking pays off while the expected queried fraction stays below about
     // 0.4 (two queried buckets per infector): I <= K / 5.
@@ -1275,6 +1321,16 @@ int pt_sim_restore(pt_sim* s) {
     k_fill_u64<<<grid_for(N, 256, 65535), 256, 0, s->stream>>>(s->claim_s, N, ~0ull);
     HIP_TRY(hipGetLastError());
     return reset_day_state(s);
+}
+
+int pt_sim_set_interventions(pt_sim* s, const pt_interventions* iv) {
+    if (!s || !iv) return fail(PT_EINVAL, "null argument");
+    if (s->setup_done) return fail(PT_ESTATE, "interventions must be set before setup");
+    Model m = s->model;
+    const int rc = apply_interventions(iv, m);
+    if (rc != PT_OK) return rc;
+    s->model = m;
+    return PT_OK;
 }
 
 int pt_sim_set_table_mode(pt_sim* s, int32_t mode) {

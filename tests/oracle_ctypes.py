@@ -53,6 +53,7 @@ def lib():
         L.or_destroy.argtypes = [C.c_void_p]
         L.or_last_error.restype = C.c_char_p
         L.or_setup.argtypes = [C.c_void_p]
+        L.or_set_interventions.argtypes = [C.c_void_p, C.c_void_p]
         L.or_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
         L.or_get_size.restype = C.c_int64
@@ -122,6 +123,10 @@ class OracleSim:
 
     def __del__(self):
         self.close()
+
+    def set_interventions(self, iv):
+        """iv: a pandemicthrust_amd.Interventions (same layout as or_interventions)."""
+        lib().or_set_interventions(self.h, C.byref(iv))
 
     def setup(self):
         if lib().or_setup(self.h) != 0:
