@@ -182,6 +182,22 @@ def replicate_bench(n_reps, plain=True):
     return out
 
 
+def ensemble_bench(scenarios, replicates, agents, days):
+    """Scenario-ensemble generation (surrogate training data): scenario
+    vectors with interventions, each run `replicates` times on the batch
+    engine, summarised as log-incidence quantiles."""
+    from pandemicthrust_amd import ensemble as E
+    X = E.sample_scenarios(scenarios, 20261203)
+    Y, info = E.run_ensemble(X, replicates, agents, days)
+    full = 4096 * 64 * agents * days / max(info["person_days_per_s"], 1.0)
+    print("ensemble: %d runs in %.2f s" % (info["runs"], info["seconds"]), file=sys.stderr, flush=True)
+    return {"workload": "%d scenarios x %d replicates, %d agents, %d days (16-dim scenario vectors with "
+                        "interventions)" % (scenarios, replicates, agents, days),
+            "runs_per_s": round(info["runs_per_s"], 2), "person_days_per_s": round(info["person_days_per_s"], 1),
+            "seconds": round(info["seconds"], 3),
+            "projected_s_4096x64": round(full, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +205,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--replicates", type=int, default=256, help="scenarios in the replicate leg (0 skips it)")
+    ap.add_argument("--ensemble", type=int, default=32, help="scenarios in the ensemble leg (0 skips it)")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched replicate leg (for profiling the batch engine)")
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
@@ -296,6 +313,8 @@ def main():
     sim.close()
     if args.replicates > 0:
         line["replicates"] = replicate_bench(args.replicates)
+    if args.ensemble > 0:
+        line["ensemble"] = ensemble_bench(args.ensemble, 8, 1000000, 120)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
