@@ -29,6 +29,8 @@ EXPORTS = [
     "pt_batch_create", "pt_batch_setup", "pt_batch_seed_infections", "pt_batch_get_info", "pt_batch_run",
     "pt_batch_generation_offsets", "pt_batch_array_bytes", "pt_batch_copy_array", "pt_batch_set_table_mode",
     "pt_batch_snapshot", "pt_batch_restore", "pt_batch_synchronize", "pt_batch_last_run_ms", "pt_batch_destroy",
+    "pt_mlp_param_count", "pt_mlp_create", "pt_mlp_set_params", "pt_mlp_get_params", "pt_mlp_set_data",
+    "pt_mlp_train", "pt_mlp_gradients", "pt_mlp_predict", "pt_mlp_last_train_ms", "pt_mlp_destroy",
 ]
 
 
@@ -130,6 +132,17 @@ def lib():
         L.pt_batch_last_run_ms.restype = C.c_double
         L.pt_batch_last_run_ms.argtypes = [C.c_void_p]
         L.pt_batch_destroy.argtypes = [C.c_void_p]
+        L.pt_mlp_param_count.restype = C.c_int64
+        L.pt_mlp_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]
+        L.pt_mlp_set_params.argtypes = [C.c_void_p, C.c_void_p]
+        L.pt_mlp_get_params.argtypes = [C.c_void_p, C.c_void_p]
+        L.pt_mlp_set_data.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        L.pt_mlp_train.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_float, C.c_void_p]
+        L.pt_mlp_gradients.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.pt_mlp_predict.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+        L.pt_mlp_last_train_ms.restype = C.c_double
+        L.pt_mlp_last_train_ms.argtypes = [C.c_void_p]
+        L.pt_mlp_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 

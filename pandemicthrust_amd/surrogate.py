@@ -1,0 +1,145 @@
+"""Epidemic surrogate: an ensemble of MLPs trained on scenario ensembles
+(include/pandemic/pt_surrogate.h), and surrogate screening of scenario
+sweeps.
+
+Inputs are scenario vectors (ensemble.SCENARIO_DIMS) scaled to [0, 1] per
+dimension; outputs are ensemble.summarise() targets flattened to 720 values
+(2 strains x 120 days x (mean, q10, q90) of log(1 + incidence)).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from .api import _check, lib
+from .ensemble import SCENARIO_DIMS
+
+IN, HIDDEN, OUT = 16, 512, 720
+DIMS = [IN, HIDDEN, HIDDEN, HIDDEN, HIDDEN, OUT]
+_LO = np.array([d[1] for d in SCENARIO_DIMS], np.float64)
+_HI = np.array([d[2] for d in SCENARIO_DIMS], np.float64)
+
+
+def normalize(X):
+    """Scenario vectors -> [0, 1] per dimension (float32)."""
+    return ((np.asarray(X, np.float64) - _LO) / (_HI - _LO)).astype(np.float32)
+
+
+def targets(Y):
+    """ensemble.run_ensemble() Y (n x 2 x 120 x 3) -> n x 720 float32."""
+    Y = np.asarray(Y)
+    assert Y.shape[1:] == (2, 120, 3), Y.shape
+    return Y.reshape(len(Y), OUT).astype(np.float32)
+
+
+def glorot(seed, members):
+    """Glorot-uniform weights and zero biases, one parameter block per member."""
+    count = int(lib().pt_mlp_param_count())
+    out = np.zeros((members, count), np.float32)
+    for k in range(members):
+        rng = np.random.default_rng([seed, k])
+        off = 0
+        for l in range(len(DIMS) - 1):
+            a = math.sqrt(6.0 / (DIMS[l] + DIMS[l + 1]))
+            n = DIMS[l] * DIMS[l + 1]
+            out[k, off:off + n] = rng.uniform(-a, a, n)
+            off += n + DIMS[l + 1]
+    return out
+
+
+def training_flops(batch, members=1):
+    """Matrix-product flops of one training step (forward, input and weight
+    gradients; the first layer needs no input gradient)."""
+    macs = sum(DIMS[l] * DIMS[l + 1] for l in range(len(DIMS) - 1))
+    macs_dx = sum(DIMS[l] * DIMS[l + 1] for l in range(1, len(DIMS) - 1))
+    return 2.0 * batch * members * (2 * macs + macs_dx)
+
+
+class Surrogate:
+    """MLP ensemble on one GPU."""
+
+    def __init__(self, members=5, batch=1024, seed=1, device=0):
+        h = C.c_void_p()
+        _check(lib().pt_mlp_create(members, batch, device, C.byref(h)))
+        self.h, self.members, self.batch = h, members, batch
+        self.count = int(lib().pt_mlp_param_count())
+        self.set_params(glorot(seed, members))
+        self.n_data = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pt_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32).reshape(self.members, self.count)
+        _check(lib().pt_mlp_set_params(self.h, p.ctypes.data))
+
+    def params(self):
+        p = np.zeros((self.members, self.count), np.float32)
+        _check(lib().pt_mlp_get_params(self.h, p.ctypes.data))
+        return p
+
+    def set_data(self, X, Y):
+        """X: n x 16 scaled inputs, Y: n x 720 targets."""
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        _check(lib().pt_mlp_set_data(self.h, X.ctypes.data, Y.ctypes.data, len(X)))
+        self.n_data = len(X)
+
+    def train(self, steps, lr=1e-3, seed=0):
+        """Adam steps over epoch permutations of the training set; returns the
+        batch losses (steps x members)."""
+        rng = np.random.default_rng(seed)
+        need = steps * self.batch
+        parts, have = [], 0
+        while have < need:
+            parts.append(rng.permutation(self.n_data))
+            have += self.n_data
+        order = np.ascontiguousarray(np.concatenate(parts)[:need], np.int32)
+        loss = np.zeros((steps, self.members), np.float32)
+        _check(lib().pt_mlp_train(self.h, order.ctypes.data, steps, C.c_float(lr), loss.ctypes.data))
+        return loss
+
+    def last_train_ms(self):
+        return float(lib().pt_mlp_last_train_ms(self.h))
+
+    def gradients(self, rows):
+        rows = np.ascontiguousarray(rows, np.int32)
+        assert len(rows) == self.batch
+        g = np.zeros((self.members, self.count), np.float32)
+        loss = np.zeros(self.members, np.float32)
+        _check(lib().pt_mlp_gradients(self.h, rows.ctypes.data, g.ctypes.data, loss.ctypes.data))
+        return g, loss
+
+    def predict(self, X):
+        """members x n x 720."""
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros((self.members, len(X), OUT), np.float32)
+        _check(lib().pt_mlp_predict(self.h, X.ctypes.data, len(X), out.ctypes.data))
+        return out
+
+
+def screen(surrogate, X, train_X, disagreement=0.5, top=0.01):
+    """Surrogate screening of raw scenario vectors X (n x 16): the ensemble
+    mean prediction, its spread, and which rows go back to the simulator —
+    members disagree (mean over outputs of the ensemble standard deviation
+    of the log-incidence mean above `disagreement`), the row lies outside the
+    box of the training inputs, or it is in the top `top` share by predicted
+    pandemic attack (sum of predicted incidence)."""
+    Xn = normalize(X)
+    P = surrogate.predict(Xn)
+    mean = P.mean(axis=0)
+    spread = P.std(axis=0).reshape(len(X), 2, 120, 3)[..., 0].mean(axis=(1, 2))
+    lo, hi = np.min(train_X, axis=0), np.max(train_X, axis=0)
+    outside = ((Xn < lo) | (Xn > hi)).any(axis=1)
+    attack = np.expm1(np.clip(mean.reshape(len(X), 2, 120, 3)[:, 0, :, 0], 0, None)).sum(axis=1)
+    k = max(1, int(math.ceil(top * len(X))))
+    top_rows = np.zeros(len(X), bool)
+    top_rows[np.argsort(-attack)[:k]] = True
+    resim = (spread > disagreement) | outside | top_rows
+    return {"mean": mean, "spread": spread, "outside": outside, "top": top_rows, "resimulate": resim,
+            "attack": attack}
