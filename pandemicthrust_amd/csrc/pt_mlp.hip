@@ -3,8 +3,9 @@
 // All matrix products go through one tiled GEMM kernel on
 // v_mfma_f32_16x16x32_bf16: BF16 operands, FP32 accumulation.  A 256-thread
 // block computes a 64 x 64 tile of C = A * B (four waves, each 32 x 32 = 2 x 2
-// MFMA tiles), staging 64 x 32 slices of A and B in LDS per K step; operands
-// may be read transposed, and the epilogue fuses bias + GELU (forward),
+// MFMA tiles), staging 64 x 64 slices of A and B in LDS per K step (16-byte
+// vector reads from global memory); operands may be read transposed, and the
+// epilogue fuses bias + GELU (forward),
 // bias (output layer), GELU backward (input gradients) or a plain FP32
 // store (weight gradients).  Every launch covers all ensemble members
 // (blockIdx.z), which train on the same batch.
@@ -35,7 +36,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kDims[PT_MLP_LAYERS + 1] = {PT_MLP_IN, PT_MLP_HIDDEN, PT_MLP_HIDDEN, PT_MLP_HIDDEN, PT_MLP_HIDDEN,
                                            PT_MLP_OUT};
-constexpr int kTM = 64, kTN = 64, kTK = 32;
+constexpr int kTM = 64, kTN = 64, kTK = 64;
 constexpr int kGemmThreads = 256;
 constexpr int kMaxWidth = PT_MLP_OUT;  // widest layer output
 
@@ -91,7 +92,8 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm(GemmArgs g) {
     const bf16* __restrict__ A = g.A + z * g.a_z;
     const bf16* __restrict__ B = g.B + z * g.b_z;
     const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * kTN;
-    // Rows of 32 + 8 bf16 (80 bytes): 16-byte aligned fragments.
+    // Tiles stored [row or column][k] with rows of kTK + 8 bf16, so each
+    // lane's eight K values are one aligned 16-byte LDS read.
     __shared__ __attribute__((aligned(16))) bf16 sA[kTM][kTK + 8];
     __shared__ __attribute__((aligned(16))) bf16 sB[kTN][kTK + 8];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -101,34 +103,56 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm(GemmArgs g) {
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const bf16 zero = (bf16)0.0f;
+    const bf16x8 zero8 = (bf16x8)(bf16)0.0f;
+    // Global reads are 16-byte vectors of 8 consecutive elements of the
+    // contiguous dimension (every stride, offset and extent here is a
+    // multiple of 8, so a vector is wholly inside or wholly outside).
+    constexpr int kVec = kTM * kTK / 8 / kGemmThreads;  // vectors per thread per operand
     for (int k0 = 0; k0 < g.K; k0 += kTK) {
-        for (int e = tid; e < kTM * kTK; e += kGemmThreads) {
-            // Consecutive threads walk the contiguous dimension of A.
-            const int mm = TA ? e % kTM : e / kTK;
-            const int kk = TA ? e / kTM : e % kTK;
-            const int m = m0 + mm, k = k0 + kk;
-            bf16 v = zero;
-            if (m < g.M && k < g.K) v = TA ? A[(int64_t)k * g.lda + m] : A[(int64_t)m * g.lda + k];
-            sA[mm][kk] = v;
-        }
-        for (int e = tid; e < kTN * kTK; e += kGemmThreads) {
-            const int nn = TB ? e / kTK : e % kTN;
-            const int kk = TB ? e % kTK : e / kTN;
-            const int n = n0 + nn, k = k0 + kk;
-            bf16 v = zero;
-            if (n < g.N && k < g.K) v = TB ? B[(int64_t)n * g.ldb + k] : B[(int64_t)k * g.ldb + n];
-            sB[nn][kk] = v;
+#pragma unroll
+        for (int q = 0; q < kVec; ++q) {
+            const int v = tid + q * kGemmThreads;
+            if (!TA) {
+                const int mm = v / (kTK / 8), kk = (v % (kTK / 8)) * 8;
+                const int m = m0 + mm, k = k0 + kk;
+                bf16x8 x = zero8;
+                if (m < g.M && k < g.K) x = *reinterpret_cast<const bf16x8*>(A + (int64_t)m * g.lda + k);
+                *reinterpret_cast<bf16x8*>(&sA[mm][kk]) = x;
+            } else {
+                const int kk = v / (kTM / 8), mm = (v % (kTM / 8)) * 8;
+                const int m = m0 + mm, k = k0 + kk;
+                bf16x8 x = zero8;
+                if (m < g.M && k < g.K) x = *reinterpret_cast<const bf16x8*>(A + (int64_t)k * g.lda + m);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) sA[mm + i][kk] = x[i];
+            }
+            if (TB) {
+                const int nn = v / (kTK / 8), kk = (v % (kTK / 8)) * 8;
+                const int n = n0 + nn, k = k0 + kk;
+                bf16x8 x = zero8;
+                if (n < g.N && k < g.K) x = *reinterpret_cast<const bf16x8*>(B + (int64_t)n * g.ldb + k);
+                *reinterpret_cast<bf16x8*>(&sB[nn][kk]) = x;
+            } else {
+                const int kk = v / (kTN / 8), nn = (v % (kTN / 8)) * 8;
+                const int n = n0 + nn, k = k0 + kk;
+                bf16x8 x = zero8;
+                if (n < g.N && k < g.K) x = *reinterpret_cast<const bf16x8*>(B + (int64_t)k * g.ldb + n);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) sB[nn + i][kk] = x[i];
+            }
         }
         __syncthreads();
-        const int kq = 8 * (lane >> 4);
 #pragma unroll
-        for (int ti = 0; ti < 2; ++ti) {
-            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sA[wm + ti * 16 + (lane & 15)][kq]);
+        for (int kc = 0; kc < kTK; kc += 32) {
+            const int kq = kc + 8 * (lane >> 4);
 #pragma unroll
-            for (int tj = 0; tj < 2; ++tj) {
-                const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sB[wn + tj * 16 + (lane & 15)][kq]);
-                acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[ti][tj], 0, 0, 0);
+            for (int ti = 0; ti < 2; ++ti) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sA[wm + ti * 16 + (lane & 15)][kq]);
+#pragma unroll
+                for (int tj = 0; tj < 2; ++tj) {
+                    const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sB[wn + tj * 16 + (lane & 15)][kq]);
+                    acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[ti][tj], 0, 0, 0);
+                }
             }
         }
         __syncthreads();
@@ -227,16 +251,27 @@ __global__ void k_loss_final(const float* __restrict__ part, int nblk, int membe
     loss[z * loss_z] = t / (float)n;
 }
 
-// Bias gradients: db[n] = sum over rows of dz[row][n], per member.
-__global__ void k_colsum(const float* __restrict__ dz, int M, int N, int64_t dz_z, float* __restrict__ db,
-                         int64_t db_z) {
+// Bias gradients: db[n] = sum over rows of dz[row][n], per member.  A block
+// of 64 columns x 16 row groups; the 16 partial sums are added in a fixed
+// order, so the result does not depend on scheduling.
+constexpr int kColCols = 64, kColGroups = 16;
+__global__ __launch_bounds__(kColCols * kColGroups) void k_colsum(const float* __restrict__ dz, int M, int N,
+                                                                 int64_t dz_z, float* __restrict__ db, int64_t db_z) {
     const int z = blockIdx.y;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+    const int c = threadIdx.x % kColCols, grp = threadIdx.x / kColCols;
+    const int n = blockIdx.x * kColCols + c;
     const float* d = dz + z * dz_z;
     float s = 0.f;
-    for (int m = 0; m < M; ++m) s += d[(int64_t)m * N + n];
-    db[z * db_z + n] = s;
+    if (n < N)
+        for (int m = grp; m < M; m += kColGroups) s += d[(int64_t)m * N + n];
+    __shared__ float s_p[kColGroups][kColCols];
+    s_p[grp][c] = s;
+    __syncthreads();
+    if (grp == 0 && n < N) {
+        float t = 0.f;
+        for (int q = 0; q < kColGroups; ++q) t += s_p[q][c];
+        db[z * db_z + n] = t;
+    }
 }
 
 // Adam on the FP32 master parameters, then their BF16 copies.
@@ -360,8 +395,8 @@ void backward(pt_mlp* m, float* loss, int64_t loss_z) {
         g.ldc = nout;
         gemm<true, false, EPI_F32>(m, g);
         // db_l
-        k_colsum<<<dim3((nout + 255) / 256, m->members), 256, 0, m->stream>>>(m->dz32[cur], B, nout, dz_z(m),
-                                                                            m->grads + m->L.b[l], P);
+        k_colsum<<<dim3((nout + kColCols - 1) / kColCols, m->members), kColCols * kColGroups, 0, m->stream>>>(
+            m->dz32[cur], B, nout, dz_z(m), m->grads + m->L.b[l], P);
         if (l == 0) break;
         // dz_{l-1} = (dz_l W_l^T) * gelu'(pre_{l-1})
         GemmArgs h = args();
