@@ -22,6 +22,14 @@ replicates: the 256-scenario set of pandemicthrust_amd/data/
 config2_scenarios.csv run to the end, setup included, once through the batch
 engine and once one simulation at a time; replicates per second both ways.
 
+ensemble: scenario vectors with interventions, each run 8 times on the batch
+engine at 1M agents x 120 days (surrogate training data generation).
+
+surrogate: a 5-member MLP ensemble trained with the BF16 MFMA kernels on
+simulated scenario ensembles (100k agents); training TFLOP/s against the
+MFMA peak, held-out accuracy, and a screening sweep (surrogate inference,
+re-simulation of flagged vectors) against simulating every vector.
+
 cpu_baseline: the CPU oracle (a sequential port of the reference semantics)
 on one host core, timed on a bounded sample of the same workload: setup plus
 the first CPU_DAYS days (setup excluded from the rate).
@@ -198,6 +206,77 @@ def ensemble_bench(scenarios, replicates, agents, days):
             "projected_s_4096x64": round(full, 1)}
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def curve_stats(logmean):
+    """Peak day, peak height and attack (sum) of incidence curves from
+    log(1 + incidence) means: ... x days -> three arrays."""
+    inc = np.expm1(np.clip(logmean, 0.0, None))
+    return inc.argmax(axis=-1), inc.max(axis=-1), inc.sum(axis=-1)
+
+
+def surrogate_bench(n_train, n_screen):
+    """Surrogate training on simulated scenario ensembles, held-out accuracy,
+    and surrogate screening of a scenario sweep with re-simulation of the
+    flagged vectors."""
+    from pandemicthrust_amd import ensemble as E
+    from pandemicthrust_amd import surrogate as S
+    agents, days, reps = 100000, 120, 8
+    Xtr, Xte = E.sample_scenarios(n_train, 101), E.sample_scenarios(64, 202)
+    Ytr, itr = E.run_ensemble(Xtr, reps, agents, days, seed0=1)
+    Yte, _ = E.run_ensemble(Xte, reps, agents, days, seed0=10 ** 6)
+    sur = S.Surrogate(members=5, batch=1024, seed=1)
+    sur.set_data(S.normalize(Xtr), S.targets(Ytr))
+    steps, train_ms = 0, 0.0
+    for n, lr in ((500, 1e-3), (500, 3e-4), (500, 1e-4)):
+        sur.train(n, lr=lr, seed=n)
+        steps += n
+        train_ms += sur.last_train_ms()
+    tflops = S.training_flops(1024, 5) * steps / (train_ms * 1e-3) / 1e12
+    P = sur.predict(S.normalize(Xte)).mean(axis=0).reshape(len(Xte), 2, days, 3)
+    rmse = float(np.sqrt(((P - Yte) ** 2).mean()))
+    pd, ph, pa = curve_stats(P[:, 0, :, 0])
+    sd, sh, sa = curve_stats(Yte[:, 0, :, 0])
+    acc = {"heldout_rmse_log_incidence": round(rmse, 4), "target_std": round(float(Yte.std()), 3),
+           "peak_day_mae": round(float(np.abs(pd - sd).mean()), 2),
+           "peak_height_rel_err": round(float(np.median(np.abs(ph - sh) / np.maximum(sh, 1.0))), 4),
+           "attack_rel_err": round(float(np.median(np.abs(pa - sa) / np.maximum(sa, 1.0))), 4)}
+    # Screening: surrogate inference on a sweep, re-simulation of the flagged rows.
+    Xs = E.sample_scenarios(n_screen, 303)
+    t0 = time.perf_counter()
+    sc = S.screen(sur, Xs, S.normalize(Xtr))
+    infer_s = time.perf_counter() - t0
+    flagged = np.nonzero(sc["resimulate"])[0]
+    chk = flagged[:64]
+    Yf, info = E.run_ensemble(Xs[chk], reps, agents, days, seed0=2 * 10 ** 6)
+    resim_s = info["seconds"] * len(flagged) / max(len(chk), 1)
+    pf = sc["mean"][chk].reshape(len(chk), 2, days, 3)
+    fd, fh, fa = curve_stats(pf[:, 0, :, 0])
+    rd, rh, ra = curve_stats(Yf[:, 0, :, 0])
+    plain_s = n_screen * reps / itr["runs_per_s"]
+    sur.close()
+    print("surrogate: %d steps in %.2f s (%.1f TFLOP/s), held-out rmse %.3f" % (steps, train_ms / 1e3, tflops, rmse),
+          file=sys.stderr, flush=True)
+    return {"workload": "5-member MLP 16-512x4-720 (GELU), BF16 MFMA / FP32 accumulate, batch 1024; trained on "
+                        "%d scenarios x %d replicates (%d agents, %d days, log-incidence mean/q10/q90)" % (
+                            n_train, reps, agents, days),
+            "train_steps": steps, "train_ms_per_step": round(train_ms / steps, 4),
+            "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": MFMA_BF16_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5)},
+            "accuracy": acc,
+            "screening": {"vectors": n_screen, "inference_s": round(infer_s, 3),
+                          "inference_per_s": round(n_screen / infer_s, 1),
+                          "flagged": int(len(flagged)), "flagged_disagreement": int((sc["spread"] > 0.5).sum()),
+                          "flagged_outside_hull": int(sc["outside"].sum()), "flagged_top": int(sc["top"].sum()),
+                          "resimulated_checked": int(len(chk)),
+                          "flagged_peak_day_mae": round(float(np.abs(fd - rd).mean()), 2) if len(chk) else None,
+                          "flagged_attack_rel_err": round(float(np.median(np.abs(fa - ra) / np.maximum(ra, 1.0))), 4)
+                          if len(chk) else None,
+                          "end_to_end_per_s": round(n_screen / (infer_s + resim_s), 1),
+                          "plain_simulation_per_s": round(n_screen / plain_s, 1)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +285,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--replicates", type=int, default=256, help="scenarios in the replicate leg (0 skips it)")
     ap.add_argument("--ensemble", type=int, default=32, help="scenarios in the ensemble leg (0 skips it)")
+    ap.add_argument("--surrogate", type=int, default=1024,
+                    help="training scenarios of the surrogate leg (0 skips it)")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched replicate leg (for profiling the batch engine)")
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
@@ -315,6 +396,8 @@ def main():
         line["replicates"] = replicate_bench(args.replicates)
     if args.ensemble > 0:
         line["ensemble"] = ensemble_bench(args.ensemble, 8, 1000000, 120)
+    if args.surrogate > 0:
+        line["surrogate"] = surrogate_bench(args.surrogate, 10000)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
