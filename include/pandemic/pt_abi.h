@@ -168,6 +168,52 @@ int pt_sim_enable_timing(pt_sim* sim, int32_t enable);
 int pt_sim_kernel_times(pt_sim* sim, double* ms_out, int64_t* launches_out, int32_t n);
 const char* pt_sim_kernel_name(int32_t k);
 
+/* Per-day kernel-class times (profiler.cu's per-function rows of each day,
+ * PandemicSim.cu:1186-1420, :2207-2224, :2738-2825): ms_out[d * C + k] =
+ * milliseconds of class k on day d, C = the class count pt_sim_kernel_times
+ * returns.  Covers the days enqueued since timing was last enabled; fills
+ * min(max_days, days) rows and returns days (ms_out may be NULL). */
+int pt_sim_day_kernel_times(pt_sim* sim, double* ms_out, int32_t max_days);
+
+/* Setup phases, timed by every handle (milliseconds), after the reference's
+ * profiled setup functions. */
+enum pt_setup_phase {
+    PT_SETUP_SCALE = 0,         /* setup_scaleSimulation :1127-1162 + setup_calculateInfectionData :2828 (host) */
+    PT_SETUP_PUSH,              /* setup_pushDeviceData :569-700: handle, stream, model buffers */
+    PT_SETUP_POPULATION_SIZE,   /* setup_calcPopulationSize_thrust :3634-3651 (K1/K2) */
+    PT_SETUP_SIZE_ARRAYS,       /* setup_sizeGlobalArrays :1047-1107 (allocation) */
+    PT_SETUP_HOUSEHOLDS,        /* setup_generateHouseholds :2552-2620 (K3/K4) */
+    PT_SETUP_WORKPLACES,        /* setup_assignWorkplaces :3812-3868 (K5/K6) */
+    PT_SETUP_STATUS,            /* setup_initializeStatusArrays :3654-3664 (K7) */
+    PT_SETUP_INITIAL_INFECTED,  /* setup_initialInfected :708-758 (seeding) */
+    PT_SETUP_PHASES
+};
+int pt_sim_setup_times(pt_sim* sim, double* ms_out, int32_t n);
+
+/* Every daily row since setup (the rows of output_infected_stats.csv,
+ * PandemicSim.cu:2771-2818, including rows restored from a checkpoint):
+ * counts_out[rows * 16] and infected_out[rows] (either may be NULL).
+ * Returns the number of rows copied, min(max_rows, rows). */
+int pt_sim_daily_rows(pt_sim* sim, int32_t* counts_out, int32_t* infected_out, int32_t max_rows);
+
+/* Checkpoint / resume at a day boundary (no counterpart in the reference,
+ * whose run is one process from setupSim to the end of runToCompletion).
+ * save writes pt_sim_checkpoint_bytes bytes into a caller buffer: the
+ * per-agent status/day/generation arrays, the loop counters (current day,
+ * rand_offset) and the daily rows so far, with a checksum.  load requires a
+ * handle created and set up from the same constants, seed and interventions
+ * (checked; the population is rebuilt by pt_sim_setup, not stored) and
+ * replaces seeding: the loop then continues exactly where the saved one
+ * stood, with results identical to an uninterrupted run. */
+int64_t pt_sim_checkpoint_bytes(pt_sim* sim);
+int pt_sim_checkpoint_save(pt_sim* sim, void* buf, int64_t bytes);
+int pt_sim_checkpoint_load(pt_sim* sim, const void* buf, int64_t bytes);
+
+/* Device memory and name for output_resource_log.csv / output_mem.csv
+ * (cudaMemGetInfo, resource_logging.cu:37, :64). */
+int pt_device_memory(int32_t device, uint64_t* free_bytes, uint64_t* total_bytes);
+int pt_device_name(int32_t device, char* buf, int32_t len);
+
 /* Blocks until the handle's stream is idle; loop time of the last
  * pt_sim_run in milliseconds (HIP events around the enqueued days). */
 int pt_sim_synchronize(pt_sim* sim);

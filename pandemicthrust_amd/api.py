@@ -31,7 +31,13 @@ EXPORTS = [
     "pt_batch_snapshot", "pt_batch_restore", "pt_batch_synchronize", "pt_batch_last_run_ms", "pt_batch_destroy",
     "pt_mlp_param_count", "pt_mlp_create", "pt_mlp_set_params", "pt_mlp_get_params", "pt_mlp_set_data",
     "pt_mlp_train", "pt_mlp_gradients", "pt_mlp_predict", "pt_mlp_last_train_ms", "pt_mlp_destroy",
+    "pt_sim_day_kernel_times", "pt_sim_setup_times", "pt_sim_daily_rows", "pt_sim_checkpoint_bytes",
+    "pt_sim_checkpoint_save", "pt_sim_checkpoint_load", "pt_device_memory", "pt_device_name",
 ]
+
+# enum pt_setup_phase (include/pandemic/pt_abi.h)
+SETUP_PHASES = ["scale", "push", "population_size", "size_arrays", "households", "workplaces", "status",
+                "initial_infected"]
 
 
 class Params(C.Structure):
@@ -143,6 +149,15 @@ def lib():
         L.pt_mlp_last_train_ms.restype = C.c_double
         L.pt_mlp_last_train_ms.argtypes = [C.c_void_p]
         L.pt_mlp_destroy.argtypes = [C.c_void_p]
+        L.pt_sim_day_kernel_times.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
+        L.pt_sim_setup_times.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
+        L.pt_sim_daily_rows.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
+        L.pt_sim_checkpoint_bytes.restype = C.c_int64
+        L.pt_sim_checkpoint_bytes.argtypes = [C.c_void_p]
+        L.pt_sim_checkpoint_save.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        L.pt_sim_checkpoint_load.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        L.pt_device_memory.argtypes = [C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.pt_device_name.argtypes = [C.c_int32, C.c_char_p, C.c_int32]
         _lib = L
     return _lib
 
@@ -157,6 +172,19 @@ def glibc_seeds(core_seed):
     out = (C.c_int32 * 4)()
     _check(lib().pt_glibc_seeds(core_seed, out))
     return list(out)
+
+
+def device_memory(device=0):
+    """(free bytes, total bytes) of a device."""
+    f, t = C.c_uint64(0), C.c_uint64(0)
+    _check(lib().pt_device_memory(device, C.byref(f), C.byref(t)))
+    return int(f.value), int(t.value)
+
+
+def device_name(device=0):
+    buf = C.create_string_buffer(256)
+    _check(lib().pt_device_name(device, buf, 256))
+    return buf.value.decode(errors="replace")
 
 
 def glibc_initial_infected(core_seed, n_people, n):
@@ -211,6 +239,46 @@ class PandemicSim:
         _check(lib().pt_sim_run(self.h, counts.ctypes.data, inf.ctypes.data, C.byref(days)))
         d = int(days.value)
         return counts[:d].copy(), inf[:d].copy()
+
+    def setup_only(self):
+        """setupSim without seeding, for load_checkpoint()."""
+        n = C.c_int64(0)
+        _check(lib().pt_sim_setup(self.h, C.byref(n)))
+        self.n_people = int(n.value)
+        return self.n_people
+
+    def daily_rows(self):
+        """Every daily row since setup (restored rows included)."""
+        counts = np.zeros((256, 16), np.int32)
+        inf = np.zeros(256, np.int32)
+        n = _check(lib().pt_sim_daily_rows(self.h, counts.ctypes.data, inf.ctypes.data, 256))
+        return counts[:n].copy(), inf[:n].copy()
+
+    def save_checkpoint(self):
+        """The loop state at the current day boundary, as bytes."""
+        n = int(lib().pt_sim_checkpoint_bytes(self.h))
+        if n < 0:
+            raise PTError("checkpoint before setup")
+        buf = np.zeros(n, np.uint8)
+        _check(lib().pt_sim_checkpoint_save(self.h, buf.ctypes.data, n))
+        return buf.tobytes()
+
+    def load_checkpoint(self, data):
+        buf = np.frombuffer(data, np.uint8)
+        _check(lib().pt_sim_checkpoint_load(self.h, buf.ctypes.data, len(buf)))
+
+    def setup_times(self):
+        ms = np.zeros(len(SETUP_PHASES), np.float64)
+        _check(lib().pt_sim_setup_times(self.h, ms.ctypes.data, len(ms)))
+        return dict(zip(SETUP_PHASES, (float(x) for x in ms)))
+
+    def day_kernel_times(self):
+        """days x classes milliseconds since timing was enabled."""
+        k = len(self.kernel_classes())
+        days = _check(lib().pt_sim_day_kernel_times(self.h, None, 0))
+        ms = np.zeros((max(days, 1), k), np.float64)
+        _check(lib().pt_sim_day_kernel_times(self.h, ms.ctypes.data, days))
+        return ms[:days]
 
     def generation_offsets(self):
         md = self.params.max_days

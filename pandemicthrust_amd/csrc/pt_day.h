@@ -82,8 +82,11 @@ void launch_household_types(const Model* d_model, int H, uint8_t* hh_types, unsi
 // N is set on the device: household offsets and people, workplaces and the
 // workplace table, statuses, empty claim words.  tmp holds max(N, H + 1)
 // uint32; ts is sized for N pairs.  Enqueued on stream, not synchronised.
+// marks, if given, are 4 events recorded before the households, after them,
+// after the workplaces and after the statuses.
 hipError_t build_population(const Model* d_model, int64_t N, int H, int W, const PopArrays& P, uint32_t* tmp,
-                            TableScratch& ts, uint64_t workplace_off, hipStream_t stream);
+                            TableScratch& ts, uint64_t workplace_off, hipStream_t stream,
+                            const hipEvent_t* marks = nullptr);
 
 // ---------------- device state ----------------
 

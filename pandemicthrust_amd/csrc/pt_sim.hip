@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -534,13 +535,22 @@ const char* kKernelNames[KC_COUNT] = {"sweep",         "plan",       "compact", 
                                       "end_day",       "mark",       "select",    "subset_sort", "setup"};
 
 // Per-class kernel timing.  mode 0: off; -1: every class; k+1: class k only.
+// Launches are also accounted to the simulated day they belong to (day is
+// set by the code that enqueues a day; -1 outside the daily loop).
 struct EventTimer : LaunchTimer {
+    struct Pending {
+        int cls;
+        size_t start;  // index of the start event (stop = start + 1)
+        int day;
+    };
     int mode = 0;
+    int day = -1;
     std::vector<hipEvent_t> pool;
-    std::vector<std::pair<int, size_t>> pending;  // class, index of the start event (stop = start + 1)
+    std::vector<Pending> pending;
     size_t used = 0;
     double ms[KC_COUNT] = {0};
     int64_t launches[KC_COUNT] = {0};
+    std::vector<double> day_ms;  // [day * KC_COUNT + class]
     hipEvent_t next() {
         if (used == pool.size()) {
             hipEvent_t e;
@@ -551,7 +561,7 @@ struct EventTimer : LaunchTimer {
     }
     bool want(int cls) const override { return mode == -1 || mode == cls + 1; }
     void events(int cls, hipEvent_t* start, hipEvent_t* stop) override {
-        pending.push_back({cls, used});
+        pending.push_back({cls, used, day});
         *start = next();
         *stop = next();
     }
@@ -559,9 +569,13 @@ struct EventTimer : LaunchTimer {
     void collect() {
         for (auto& p : pending) {
             float t = 0.f;
-            if (hipEventElapsedTime(&t, pool[p.second], pool[p.second + 1]) == hipSuccess) {
-                ms[p.first] += t;
-                launches[p.first] += 1;
+            if (hipEventElapsedTime(&t, pool[p.start], pool[p.start + 1]) == hipSuccess) {
+                ms[p.cls] += t;
+                launches[p.cls] += 1;
+                if (p.day >= 0) {
+                    if (day_ms.size() < (size_t)(p.day + 1) * KC_COUNT) day_ms.resize((size_t)(p.day + 1) * KC_COUNT, 0.0);
+                    day_ms[(size_t)p.day * KC_COUNT + p.cls] += t;
+                }
             }
         }
         pending.clear();
@@ -569,6 +583,8 @@ struct EventTimer : LaunchTimer {
     }
     void reset() {
         pending.clear();
+        day_ms.clear();
+        day = -1;
         used = 0;
         for (int i = 0; i < KC_COUNT; ++i) {
             ms[i] = 0;
@@ -678,22 +694,26 @@ void launch_household_types(const Model* d_model, int H, uint8_t* hh_types, unsi
 }
 
 hipError_t build_population(const Model* d_model, int64_t N, int H, int W, const PopArrays& P, uint32_t* tmp,
-                            TableScratch& ts, uint64_t workplace_off, hipStream_t st) {
+                            TableScratch& ts, uint64_t workplace_off, hipStream_t st, const hipEvent_t* marks) {
+    if (marks) (void)hipEventRecord(marks[0], st);
     // K3/K4: household offsets by exclusive scan, then people (PandemicSim.cu:2573-2613).
     k_household_sizes<<<grid_for((int64_t)H + 1, 256, 65535), 256, 0, st>>>(d_model, P.hh_types, tmp);
     hipError_t e = exclusive_scan_u32(tmp, (uint32_t*)P.hh_off, (int64_t)H + 1, ts.scan_part, nullptr, st);
     if (e != hipSuccess) return e;
     k_generate_households<<<grid_for(H, 256, 65535), 256, 0, st>>>(d_model, P.hh_types, P.hh_off, P.household, P.age);
+    if (marks) (void)hipEventRecord(marks[1], st);
     // K5/K6: workplaces, then the workplace table (PandemicSim.cu:3812-3862).
     k_assign_workplaces<<<grid_for(N / 4 + 1, 256, 65535), 256, 0, st>>>(d_model, workplace_off, P.age, tmp);
     e = build_table(ts, tmp, nullptr, N, nullptr, W, ID_IDENTITY, (int32_t)N, P.wp_people, P.wp_off, nullptr, st,
                     nullptr);
     if (e != hipSuccess) return e;
+    if (marks) (void)hipEventRecord(marks[2], st);
     // K7: statuses; claim words start empty.
     k_init_status<<<grid_for(N, 256, 65535), 256, 0, st>>>(d_model, N, P.st_p, P.st_s, P.dy_p, P.dy_s, P.gn_p,
                                                            P.gn_s);
     k_fill_u64<<<grid_for(N, 256, 65535), 256, 0, st>>>(P.claim_p, N, ~0ull);
     k_fill_u64<<<grid_for(N, 256, 65535), 256, 0, st>>>(P.claim_s, N, ~0ull);
+    if (marks) (void)hipEventRecord(marks[3], st);
     return hipGetLastError();
 }
 
@@ -740,11 +760,22 @@ struct pt_sim {
     int64_t last_table_n = 0;
     EventTimer timer;
     hipEvent_t run_begin = nullptr, run_end = nullptr;
+    hipEvent_t setup_ev[6] = {};               // population size (2), build_population marks (4)
+    double setup_ms[PT_SETUP_PHASES] = {0};
     double last_run_ms = 0;
     int grid_contacts = 1024, grid_gen = 2048;
 };
 
 namespace {
+
+double ms_between(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+double event_ms(hipEvent_t a, hipEvent_t b) {
+    float t = 0.f;
+    return hipEventElapsedTime(&t, a, b) == hipSuccess ? (double)t : 0.0;
+}
 
 // Buffers sized by the population (allocated in pt_sim_setup).
 void free_population(pt_sim* s) {
@@ -812,6 +843,7 @@ AgentArrays agent_arrays(pt_sim* s, int32_t* trace_victim, uint8_t* trace_type) 
 int enqueue_day(pt_sim* s, int d) {
     const bool weekend = (d % 7) >= 5;
     hipStream_t stc = s->stream;
+    s->timer.day = d;
     LaunchTimer* tm = &s->timer;
     const int N = (int)s->N;
     DayState* st = s->d_state;
@@ -885,6 +917,7 @@ int enqueue_day(pt_sim* s, int d) {
     s->last_weekend = weekend ? 1 : 0;
     s->last_table_n = n;
     s->host_day = d + 1;
+    s->timer.day = -1;
     return 0;
 }
 
@@ -987,11 +1020,14 @@ int pt_glibc_initial_infected(int32_t core_seed, int32_t n_people, int32_t n, in
 int pt_sim_create(const pt_params* p, const int32_t seed_key[2], int32_t device, pt_sim** out) {
     if (!p || !seed_key || !out) return fail(PT_EINVAL, "null argument");
     *out = nullptr;
+    const auto t0 = std::chrono::steady_clock::now();
     Model m{};
     const int rc = build_model(p, seed_key, m);
     if (rc != PT_OK) return rc;
+    const auto t1 = std::chrono::steady_clock::now();
     HIP_TRY(hipSetDevice(device));
     pt_sim* s = new pt_sim();
+    s->setup_ms[PT_SETUP_SCALE] = ms_between(t0, t1);
     s->device = device;
     s->params = *p;
     s->model = m;
@@ -1016,6 +1052,8 @@ int pt_sim_create(const pt_params* p, const int32_t seed_key[2], int32_t device,
     }
     (void)hipEventCreate(&s->run_begin);
     (void)hipEventCreate(&s->run_end);
+    for (auto& e : s->setup_ev) (void)hipEventCreate(&e);
+    s->setup_ms[PT_SETUP_PUSH] = ms_between(t1, std::chrono::steady_clock::now());
     *out = s;
     return PT_OK;
 }
@@ -1032,10 +1070,13 @@ int pt_sim_setup(pt_sim* s, int64_t* n_people_out) {
     HIP_TRY(hipMemsetAsync(s->d_scratch64, 0, sizeof(unsigned long long), st));
     m.N = 0;
     HIP_TRY(hipMemcpyAsync(s->d_model, &m, sizeof m, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(s->setup_ev[0], st));
     launch_household_types(s->d_model, H, s->hh_types, s->d_scratch64, st);
+    HIP_TRY(hipEventRecord(s->setup_ev[1], st));
     unsigned long long npeople = 0;
     HIP_TRY(hipMemcpyAsync(&npeople, s->d_scratch64, sizeof npeople, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    const auto t_sized0 = std::chrono::steady_clock::now();
     if (npeople == 0 || npeople > (unsigned long long)(0x7FFFFFFF / 8))
         return fail(PT_EINVAL, "population %llu outside the supported range", npeople);
     const int64_t N = (int64_t)npeople;
@@ -1072,15 +1113,22 @@ int pt_sim_setup(pt_sim* s, int64_t* n_people_out) {
         free_population(s);
         return fail(PT_ENOMEM, "table scratch allocation failed");
     }
+    const double sizing_ms = ms_between(t_sized0, std::chrono::steady_clock::now());
     // K3-K7 (the keys buffer is the scratch; the rand offset after the
     // households is the workplace counter, PandemicSim.cu:2570-2571, :3817).
     s->workplace_off = (uint64_t)(H / 4);
     s->rand_after_setup = s->workplace_off + (uint64_t)(N / 4);
     const PopArrays P{s->hh_types, s->hh_off, s->household, s->age, s->wp_off, s->wp_people, s->st_p, s->st_s,
                       s->dy_p, s->dy_s, s->gn_p, s->gn_s, s->claim_p, s->claim_s};
-    hipError_t e = build_population(s->d_model, N, H, s->W, P, s->keys, s->tables, s->workplace_off, st);
+    hipError_t e = build_population(s->d_model, N, H, s->W, P, s->keys, s->tables, s->workplace_off, st,
+                                    s->setup_ev + 2);
     if (e != hipSuccess) return fail(PT_EDEVICE, "population setup: %s", hipGetErrorString(e));
     HIP_TRY(hipStreamSynchronize(st));
+    s->setup_ms[PT_SETUP_POPULATION_SIZE] = event_ms(s->setup_ev[0], s->setup_ev[1]);
+    s->setup_ms[PT_SETUP_SIZE_ARRAYS] = sizing_ms;
+    s->setup_ms[PT_SETUP_HOUSEHOLDS] = event_ms(s->setup_ev[2], s->setup_ev[3]);
+    s->setup_ms[PT_SETUP_WORKPLACES] = event_ms(s->setup_ev[3], s->setup_ev[4]);
+    s->setup_ms[PT_SETUP_STATUS] = event_ms(s->setup_ev[4], s->setup_ev[5]);
     s->setup_done = true;
     s->seeded = false;
     if (reset_day_state(s)) return PT_EDEVICE;
@@ -1102,6 +1150,7 @@ int pt_sim_seed_infections(pt_sim* s, const int32_t* ids, int32_t n_p, int32_t n
             if (sorted[i] == sorted[i - 1]) return fail(PT_EINVAL, "duplicate initial infected id %d", sorted[i]);
     }
     HIP_TRY(hipSetDevice(s->device));
+    const auto t0 = std::chrono::steady_clock::now();
     if (n > 0) {
         int32_t* d_ids = nullptr;
         if (dev_alloc(&d_ids, n)) return PT_ENOMEM;
@@ -1113,7 +1162,9 @@ int pt_sim_seed_infections(pt_sim* s, const int32_t* ids, int32_t n_p, int32_t n
         if (e != hipSuccess) return fail(PT_EDEVICE, "seed infections: %s", hipGetErrorString(e));
     }
     s->seeded = true;
-    return reset_day_state(s);
+    const int rc = reset_day_state(s);
+    s->setup_ms[PT_SETUP_INITIAL_INFECTED] = ms_between(t0, std::chrono::steady_clock::now());
+    return rc;
 }
 
 int pt_sim_step(pt_sim* s, int32_t counts16_out[16], int32_t* infected_out) {
@@ -1323,6 +1374,196 @@ int pt_sim_restore(pt_sim* s) {
     return reset_day_state(s);
 }
 
+// ---------------- checkpoint / resume ----------------
+//
+// Buffer layout (host byte order, little-endian on every supported host):
+//   CkptHeader
+//   Model          the handle's model after setup (scales, tables, seed
+//                  key, interventions, N): a checkpoint loads only into a
+//                  handle built from the same inputs
+//   DayState       the loop's device counters at a day boundary
+//   int32 counts[256 * 16], infected[256], tab_full[256], tab_pairs[256]
+//   uint8 status_p[N], status_s[N], day_p[N], day_s[N], gen_p[N], gen_s[N]
+// The population (households, ages, workplaces) is not stored: it is a
+// function of the inputs, rebuilt by pt_sim_setup.  Claim words are empty
+// at every day boundary and are cleared on load.
+namespace {
+
+struct CkptHeader {
+    char magic[8];
+    uint32_t version;
+    uint32_t model_bytes;
+    uint32_t state_bytes;
+    uint32_t rows_capacity;
+    int64_t n_people;
+    uint64_t checksum;  // over everything after the header
+};
+constexpr char kCkptMagic[8] = {'P', 'T', 'S', 'T', 'A', 'T', 'E', '1'};
+constexpr uint32_t kCkptVersion = 1;
+constexpr int kCkptRows = 256;
+constexpr int64_t kCkptLogBytes = (int64_t)kCkptRows * (16 + 1 + 1 + 1) * 4;
+
+int64_t ckpt_fixed_bytes() { return (int64_t)sizeof(CkptHeader) + sizeof(Model) + sizeof(DayState) + kCkptLogBytes; }
+
+// FNV-1a over 64-bit words (then the tail bytes).
+uint64_t ckpt_hash(const uint8_t* p, int64_t n) {
+    uint64_t h = 0xcbf29ce484222325ull;
+    int64_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        uint64_t w;
+        memcpy(&w, p + i, 8);
+        h = (h ^ w) * 0x100000001b3ull;
+    }
+    for (; i < n; ++i) h = (h ^ p[i]) * 0x100000001b3ull;
+    return h;
+}
+
+}  // namespace
+
+int64_t pt_sim_checkpoint_bytes(pt_sim* s) {
+    if (!s || !s->setup_done) return -1;
+    return ckpt_fixed_bytes() + 6 * s->N;
+}
+
+int pt_sim_checkpoint_save(pt_sim* s, void* buf, int64_t bytes) {
+    if (!s || !buf) return fail(PT_EINVAL, "null argument");
+    if (!s->setup_done || !s->seeded) return fail(PT_ESTATE, "checkpoint before setup and seeding");
+    const int64_t need = pt_sim_checkpoint_bytes(s);
+    if (bytes < need) return fail(PT_EINVAL, "checkpoint buffer holds %lld bytes, %lld needed", (long long)bytes,
+                                  (long long)need);
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    uint8_t* out = static_cast<uint8_t*>(buf);
+    CkptHeader hd{};
+    memcpy(hd.magic, kCkptMagic, 8);
+    hd.version = kCkptVersion;
+    hd.model_bytes = sizeof(Model);
+    hd.state_bytes = sizeof(DayState);
+    hd.rows_capacity = kCkptRows;
+    hd.n_people = s->N;
+    uint8_t* p = out + sizeof hd;
+    memcpy(p, &s->model, sizeof(Model));
+    p += sizeof(Model);
+    HIP_TRY(hipMemcpy(p, s->d_state, sizeof(DayState), hipMemcpyDeviceToHost));
+    p += sizeof(DayState);
+    HIP_TRY(hipMemcpy(p, s->d_counts, 4 * kCkptRows * 16, hipMemcpyDeviceToHost));
+    p += 4 * kCkptRows * 16;
+    HIP_TRY(hipMemcpy(p, s->d_inf, 4 * kCkptRows, hipMemcpyDeviceToHost));
+    p += 4 * kCkptRows;
+    HIP_TRY(hipMemcpy(p, s->d_tab_full, 4 * kCkptRows, hipMemcpyDeviceToHost));
+    p += 4 * kCkptRows;
+    HIP_TRY(hipMemcpy(p, s->d_tab_pairs, 4 * kCkptRows, hipMemcpyDeviceToHost));
+    p += 4 * kCkptRows;
+    const uint8_t* arr[6] = {s->st_p, s->st_s, s->dy_p, s->dy_s, s->gn_p, s->gn_s};
+    for (int i = 0; i < 6; ++i) {
+        HIP_TRY(hipMemcpy(p, arr[i], (size_t)s->N, hipMemcpyDeviceToHost));
+        p += s->N;
+    }
+    hd.checksum = ckpt_hash(out + sizeof hd, need - (int64_t)sizeof hd);
+    memcpy(out, &hd, sizeof hd);
+    return PT_OK;
+}
+
+int pt_sim_checkpoint_load(pt_sim* s, const void* buf, int64_t bytes) {
+    if (!s || !buf) return fail(PT_EINVAL, "null argument");
+    if (!s->setup_done) return fail(PT_ESTATE, "checkpoint load before setup");
+    const uint8_t* in = static_cast<const uint8_t*>(buf);
+    if (bytes < (int64_t)sizeof(CkptHeader)) return fail(PT_EINVAL, "checkpoint truncated (%lld bytes)", (long long)bytes);
+    CkptHeader hd;
+    memcpy(&hd, in, sizeof hd);
+    if (memcmp(hd.magic, kCkptMagic, 8) != 0) return fail(PT_EINVAL, "not a checkpoint (bad magic)");
+    if (hd.version != kCkptVersion || hd.model_bytes != sizeof(Model) || hd.state_bytes != sizeof(DayState) ||
+        hd.rows_capacity != (uint32_t)kCkptRows)
+        return fail(PT_EINVAL, "checkpoint written by an incompatible library version");
+    if (hd.n_people != s->N)
+        return fail(PT_EINVAL, "checkpoint holds %lld agents, this simulation %lld", (long long)hd.n_people,
+                    (long long)s->N);
+    const int64_t need = pt_sim_checkpoint_bytes(s);
+    if (bytes < need) return fail(PT_EINVAL, "checkpoint truncated (%lld of %lld bytes)", (long long)bytes,
+                                  (long long)need);
+    if (ckpt_hash(in + sizeof hd, need - (int64_t)sizeof hd) != hd.checksum)
+        return fail(PT_EINVAL, "checkpoint checksum mismatch (corrupt file)");
+    const uint8_t* p = in + sizeof hd;
+    if (memcmp(p, &s->model, sizeof(Model)) != 0)
+        return fail(PT_EINVAL, "checkpoint was written for other constants, seed or interventions");
+    p += sizeof(Model);
+    DayState st;
+    memcpy(&st, p, sizeof st);
+    p += sizeof(DayState);
+    if (st.day < 0 || st.day > 256 || st.days_run < 0 || st.days_run > kCkptRows || st.max_days != s->params.max_days)
+        return fail(PT_EINVAL, "checkpoint day state out of range");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(s->d_state, &st, sizeof st, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->d_counts, p, 4 * kCkptRows * 16, hipMemcpyHostToDevice));
+    p += 4 * kCkptRows * 16;
+    HIP_TRY(hipMemcpy(s->d_inf, p, 4 * kCkptRows, hipMemcpyHostToDevice));
+    p += 4 * kCkptRows;
+    HIP_TRY(hipMemcpy(s->d_tab_full, p, 4 * kCkptRows, hipMemcpyHostToDevice));
+    p += 4 * kCkptRows;
+    HIP_TRY(hipMemcpy(s->d_tab_pairs, p, 4 * kCkptRows, hipMemcpyHostToDevice));
+    p += 4 * kCkptRows;
+    uint8_t* arr[6] = {s->st_p, s->st_s, s->dy_p, s->dy_s, s->gn_p, s->gn_s};
+    for (int i = 0; i < 6; ++i) {
+        HIP_TRY(hipMemcpy(arr[i], p, (size_t)s->N, hipMemcpyHostToDevice));
+        p += s->N;
+    }
+    k_fill_u64<<<grid_for(s->N, 256, 65535), 256, 0, s->stream>>>(s->claim_p, s->N, ~0ull);
+    k_fill_u64<<<grid_for(s->N, 256, 65535), 256, 0, s->stream>>>(s->claim_s, s->N, ~0ull);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->seeded = true;
+    s->host_day = 0;
+    s->last_weekend = -1;
+    return PT_OK;
+}
+
+int pt_sim_daily_rows(pt_sim* s, int32_t* counts_out, int32_t* infected_out, int32_t max_rows) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    if (!s->setup_done) return fail(PT_ESTATE, "daily rows before setup");
+    HIP_TRY(hipSetDevice(s->device));
+    DayState h;
+    if (read_state(s, &h)) return PT_EDEVICE;
+    const int rows = std::max(0, std::min(max_rows, h.days_run));
+    if (rows > 0 && counts_out)
+        HIP_TRY(hipMemcpy(counts_out, s->d_counts, sizeof(int32_t) * 16 * rows, hipMemcpyDeviceToHost));
+    if (rows > 0 && infected_out)
+        HIP_TRY(hipMemcpy(infected_out, s->d_inf, sizeof(int32_t) * rows, hipMemcpyDeviceToHost));
+    return rows;
+}
+
+int pt_sim_setup_times(pt_sim* s, double* ms_out, int32_t n) {
+    if (!s || !ms_out) return fail(PT_EINVAL, "null argument");
+    for (int k = 0; k < n && k < PT_SETUP_PHASES; ++k) ms_out[k] = s->setup_ms[k];
+    return PT_SETUP_PHASES;
+}
+
+int pt_sim_day_kernel_times(pt_sim* s, double* ms_out, int32_t max_days) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    const int days = (int)(s->timer.day_ms.size() / KC_COUNT);
+    const int n = std::max(0, std::min(days, max_days));
+    if (ms_out && n > 0) memcpy(ms_out, s->timer.day_ms.data(), sizeof(double) * KC_COUNT * n);
+    return days;
+}
+
+int pt_device_memory(int32_t device, uint64_t* free_bytes, uint64_t* total_bytes) {
+    HIP_TRY(hipSetDevice(device));
+    size_t f = 0, t = 0;
+    HIP_TRY(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return PT_OK;
+}
+
+int pt_device_name(int32_t device, char* buf, int32_t len) {
+    if (!buf || len <= 0) return fail(PT_EINVAL, "null buffer");
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    const char* name = prop.name[0] ? prop.name : prop.gcnArchName;
+    snprintf(buf, (size_t)len, "%s", name);
+    return PT_OK;
+}
+
 int pt_sim_set_interventions(pt_sim* s, const pt_interventions* iv) {
     if (!s || !iv) return fail(PT_EINVAL, "null argument");
     if (s->setup_done) return fail(PT_ESTATE, "interventions must be set before setup");
@@ -1395,6 +1636,8 @@ void pt_sim_destroy(pt_sim* s) {
     dev_free(s->d_scratch64);
     if (s->run_begin) (void)hipEventDestroy(s->run_begin);
     if (s->run_end) (void)hipEventDestroy(s->run_end);
+    for (auto e : s->setup_ev)
+        if (e) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
