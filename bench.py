@@ -63,12 +63,13 @@ def algorithmic_bytes(N, W, lay, inf, counts, full, pairs):
     the per-day model B_wd = 43N + 89I + 3X + 8W + 4, B_we = 62N + 62I + 3X +
     40W + 4 (terminal I = 0 day: 2N).
 
-    full[d], pairs[d]: the day's errand table (every bucket, n = 2N or 3N
-    pairs; or the queried buckets only, pairs[d] selected pairs) as the
-    engine logged it.  Full-table days account to gen/sort/offsets, subset
-    days to select/subset_sort.  Per pair and radix pass: upsweep reads the
-    key (4 B); downsweep reads key (+ id when ids are explicit or after the
-    first pass) and writes key and id (12 or 16 B)."""
+    full[d], pairs[d]: the day's errand table (every bucket: n = 2N or 3N
+    keys, pairs[d] of them kept; or the queried buckets only, pairs[d]
+    selected pairs) as the engine logged it.  Full-table days account to
+    gen/sort/offsets, subset days to select/subset_sort.  Per radix pass:
+    upsweep reads the keys (4 B each); the first downsweep reads n keys and
+    writes key and id of the kept pairs (4n + 8k), later ones read and write
+    key and id (16k)."""
     per = {}
 
     def add(k, b):
@@ -99,11 +100,14 @@ def algorithmic_bytes(N, W, lay, inf, counts, full, pairs):
             loop += 43 * N + 89 * I + 3 * X + 8 * W + 4
         if d < len(full) and full[d]:
             n = 3 * N if weekend else 2 * N
+            # pairs kept: the weekday table leaves out the never-queried
+            # hour-1 bucket (the first pass reads every key, writes the rest)
+            k = int(pairs[d]) if d < len(pairs) and int(pairs[d]) > 0 else n
             add("gen_weekend" if weekend else "gen_weekday", 4 * n + (0 if weekend else N))
             for p in range(passes):
-                add("sort_upsweep", 4 * n)
-                add("sort_downsweep", n * (4 + (4 if p else 0) + 4 + 4))
-            add("offsets", 4 * n + 4 * (buckets + 1))
+                add("sort_upsweep", 4 * (k if p else n))
+                add("sort_downsweep", (4 * n + 8 * k) if p == 0 else 16 * k)
+            add("offsets", 4 * k + 4 * (buckets + 1))
         elif d < len(full):
             m = int(pairs[d])
             add("select", (0 if weekend else N) + 8 * m + 16 * m)   # ages; staged pairs; compaction

@@ -580,7 +580,7 @@ __device__ __forceinline__ void weekday_contacts(const Model& M, const AgentArra
     // workplace (device_recalcWorkplace PandemicSim.cu:3872-3890)
     const U4 uw = draw4(workplace_off + (uint64_t)(idx / 4), M.key0, M.key1);
     uint8_t a2 = myage;
-    const int32_t wp = assign_workplace(M, uw.w[idx % 4], &a2);
+    const int32_t wp = assign_workplace(M, word(uw, idx % 4), &a2);
     const int32_t woff = A.wp_off[wp];
     const int32_t wcnt = A.wp_off[wp + 1] - woff;
     const int wpd = workplace_max_contacts(M, wp);
@@ -588,8 +588,8 @@ __device__ __forceinline__ void weekday_contacts(const Model& M, const AgentArra
     const U4 ue = draw4(eoff + (uint64_t)(idx / 2), M.key0, M.key1);
     const int rs = 2 * (idx % 2);
     int32_t d1, d2;
-    weekday_dests(M, ue.w[rs], ue.w[rs + 1], adult, d1, d2);
-    const int prof = adult ? (int)((ue.w[rs] ^ ue.w[rs + 1]) % 3u) : 3;
+    weekday_dests(M, word(ue, rs), word(ue, rs + 1), adult, d1, d2);
+    const int prof = adult ? (int)((word(ue, rs) ^ word(ue, rs + 1)) % 3u) : 3;
     const int h0 = M.wd_assign[prof][0], h1 = M.wd_assign[prof][1];
     uint32_t k0, k1;
     weekday_keys(M, d1, d2, k0, k1);
@@ -696,10 +696,10 @@ __device__ __forceinline__ void mark_weekday(const Model& M, const AgentArrays& 
     const U4 ue = draw4(eoff + (uint64_t)(idx / 2), M.key0, M.key1);
     const int rs = 2 * (idx % 2);
     int32_t d1, d2;
-    weekday_dests(M, ue.w[rs], ue.w[rs + 1], adult, d1, d2);
+    weekday_dests(M, word(ue, rs), word(ue, rs + 1), adult, d1, d2);
     uint32_t k0, k1;
     weekday_keys(M, d1, d2, k0, k1);
-    const int prof = adult ? (int)((ue.w[rs] ^ ue.w[rs + 1]) % 3u) : 3;
+    const int prof = adult ? (int)((word(ue, rs) ^ word(ue, rs + 1)) % 3u) : 3;
     if (M.wd_assign[prof][0] > 0) set_mark(mark, (int32_t)k0, M.wd_K, kbias);
     if (M.wd_assign[prof][1] > 0) set_mark(mark, (int32_t)k1, M.wd_K, kbias);
 }

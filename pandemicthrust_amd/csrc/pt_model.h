@@ -80,6 +80,12 @@ __host__ __device__ __forceinline__ uint32_t rotl32(uint32_t x, unsigned n) {
 // Threefry2x64, 20 rounds, counter {c, c}: the reference always passes the
 // same 64-bit offset in both counter words (e.g. PandemicSim.cu:2340).
 struct U4 { uint32_t w[4]; };
+
+// Word i (0..3) of a draw, selected without dynamic indexing (an indexed
+// read of a register array is placed in scratch memory).
+__host__ __device__ __forceinline__ uint32_t word(const U4& u, int i) {
+    return i == 0 ? u.w[0] : (i == 1 ? u.w[1] : (i == 2 ? u.w[2] : u.w[3]));
+}
 __device__ __forceinline__ U4 draw4(uint64_t c, uint64_t k0, uint64_t k1) {
     const uint64_t k2 = 0x1BD11BDAA9FC1A22ull ^ k0 ^ k1;
     uint64_t x0 = c + k0, x1 = c + k1;

@@ -465,12 +465,15 @@ __global__ __launch_bounds__(kContactThreads) void k_contacts(const Model* __res
 
 // Also logs the day's errand table: full (every pair, n_full) or the subset
 // of queried buckets (n_sel pairs).
+// tab_pairs logs the pairs of the day's table: n_kept (when set) or n_full
+// on full-table days, the selected pairs on subset days.
 __global__ void k_end_day(DayState* st, int32_t* __restrict__ tab_full, int32_t* __restrict__ tab_pairs,
-                          int32_t n_full) {
+                          int32_t n_full, const uint32_t* __restrict__ n_kept) {
     if (st->done) return;
     const int d = st->day;
     tab_full[d] = st->skip_full == 0 ? 1 : 0;
-    tab_pairs[d] = st->skip_full == 0 ? n_full : (st->skip_subset == 0 ? (int32_t)st->n_sel : 0);
+    tab_pairs[d] = st->skip_full == 0 ? (n_kept ? (int32_t)*n_kept : n_full)
+                                      : (st->skip_subset == 0 ? (int32_t)st->n_sel : 0);
     st->day += 1;
     st->n_claimed[0] = 0;
     st->n_claimed[1] = 0;
@@ -881,8 +884,10 @@ int enqueue_day(pt_sim* s, int d) {
     else
         launch(tm, KC_GEN_WEEKDAY, k_gen_weekday, dim3(s->grid_gen), dim3(256), stc, M, st, (const uint8_t*)s->age,
                s->keys);
+    // Weekday: the last bucket collects the hour-1 keys nobody queries
+    // (children stay at their after-school site), left out of the table.
     hipError_t e = build_table(s->tables, s->keys, nullptr, n, nullptr, K, mode, N, s->er_people, s->er_off,
-                               &st->skip_full, stc, tm);
+                               &st->skip_full, stc, tm, weekend ? kNoDrop : (uint32_t)(K - 1));
     if (e != hipSuccess) return fail(PT_EDEVICE, "table build: %s", hipGetErrorString(e));
     // Subset path: only pairs of queried buckets, explicit ids.
     const int nb = (int)((s->N + (weekend ? kSelAgentsWe : kSelAgentsWd) - 1) / (weekend ? kSelAgentsWe : kSelAgentsWd));
@@ -911,7 +916,8 @@ int enqueue_day(pt_sim* s, int d) {
                infected, A, s->list_p, s->list_s, s->workplace_off);
     launch(tm, KC_APPLY, k_apply, dim3(s->grid_contacts), dim3(256), stc, (const DayState*)st, s->claim_p, s->claim_s,
            (const int32_t*)s->list_p, (const int32_t*)s->list_s, s->st_p, s->st_s, s->dy_p, s->dy_s, s->gn_p, s->gn_s);
-    launch(tm, KC_END_DAY, k_end_day, dim3(1), dim3(1), stc, st, s->d_tab_full, s->d_tab_pairs, (int32_t)n);
+    launch(tm, KC_END_DAY, k_end_day, dim3(1), dim3(1), stc, st, s->d_tab_full, s->d_tab_pairs, (int32_t)n,
+           weekend ? (const uint32_t*)nullptr : (const uint32_t*)s->tables.n_kept);
     e = hipGetLastError();
     if (e != hipSuccess) return fail(PT_EDEVICE, "day launch: %s", hipGetErrorString(e));
     s->last_weekend = weekend ? 1 : 0;

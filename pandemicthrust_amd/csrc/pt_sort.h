@@ -79,6 +79,7 @@ struct TableScratch {
     int32_t* vals_b = nullptr;   // n_cap
     uint32_t* hist = nullptr;    // 256 * kMaxSortBlocks
     uint32_t* scan_part = nullptr;
+    uint32_t* n_kept = nullptr;  // pairs kept by a first pass that dropped a key (device)
     int64_t n_cap = 0;
     // Keys come in long runs of equal high bits (tables of many simulations
     // side by side): count digits with wave-aggregated atomics.
@@ -88,13 +89,18 @@ struct TableScratch {
 int table_scratch_alloc(TableScratch& s, int64_t n_cap);
 void table_scratch_free(TableScratch& s);
 
+constexpr uint32_t kNoDrop = 0xFFFFFFFFu;
+
 // Builds the table.  keys is clobbered.  vals (may be null) holds explicit
 // ids in input order; when null the ids follow from mode.  When n_dev is
 // set it holds the pair count on the device and n is only the capacity.
 // done (device int, may be null) turns every kernel into a no-op when set.
+// drop_key = K - 1 leaves the pairs of the last bucket out of the table (a
+// bucket nobody queries): that bucket comes out empty, offsets[K - 1] =
+// offsets[K] = pairs kept, and ts.n_kept holds that count on the device.
 hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int64_t n, const uint32_t* n_dev,
                        int32_t K, IdMode mode, int32_t N, int32_t* people_out, int32_t* offsets_out,
-                       const int32_t* done, hipStream_t stream, LaunchTimer* tm);
+                       const int32_t* done, hipStream_t stream, LaunchTimer* tm, uint32_t drop_key = kNoDrop);
 
 // Number of radix passes build_table uses for K buckets.
 int radix_passes(int32_t K);

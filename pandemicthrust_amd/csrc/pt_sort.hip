@@ -79,12 +79,17 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, u
 }
 
 // Per-block digit histogram of the block's chunk.
+// Keys equal to drop are left out (not counted, not written by the
+// matching downsweep).  hist[nblk << BITS] is set to 0 so that the scan of
+// nblk * RADIX + 1 entries leaves the pass's total there.
 template <int BITS, bool MATCH>
 __global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t* __restrict__ keys, int64_t n_host,
                                                       const uint32_t* __restrict__ n_dev, int shift,
-                                                      uint32_t* __restrict__ hist, int nblk, const int32_t* done) {
+                                                      uint32_t* __restrict__ hist, int nblk, const int32_t* done,
+                                                      uint32_t drop) {
     if (is_done(done)) return;
     constexpr int RADIX = 1 << BITS;
+    if (blockIdx.x == 0 && threadIdx.x == 0) hist[(int64_t)nblk << BITS] = 0;
     __shared__ uint32_t s_h[kWaves][RADIX];
     const int tid = threadIdx.x, wave = tid >> 6;
     for (int i = tid; i < kWaves * RADIX; i += kThreads) (&s_h[0][0])[i] = 0;
@@ -110,7 +115,7 @@ __global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t* __restrict
             }
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                const bool valid = q * 4 + c < end;
+                const bool valid = q * 4 + c < end && kk[c] != drop;
                 const uint32_t d = (kk[c] >> shift) & (RADIX - 1);
                 const uint64_t act = __ballot(valid);
                 const int first = act ? __ffsll((unsigned long long)act) - 1 : 0;  // lowest valid lane
@@ -127,12 +132,13 @@ __global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t* __restrict
         for (int64_t q = begin / 4 + tid; q * 4 < end; q += kThreads) {
             if (q * 4 + 3 < end) {
                 const uint4 v = k4[q];
-                atomicAdd(&s_h[wave][(v.x >> shift) & (RADIX - 1)], 1u);
-                atomicAdd(&s_h[wave][(v.y >> shift) & (RADIX - 1)], 1u);
-                atomicAdd(&s_h[wave][(v.z >> shift) & (RADIX - 1)], 1u);
-                atomicAdd(&s_h[wave][(v.w >> shift) & (RADIX - 1)], 1u);
+                if (v.x != drop) atomicAdd(&s_h[wave][(v.x >> shift) & (RADIX - 1)], 1u);
+                if (v.y != drop) atomicAdd(&s_h[wave][(v.y >> shift) & (RADIX - 1)], 1u);
+                if (v.z != drop) atomicAdd(&s_h[wave][(v.z >> shift) & (RADIX - 1)], 1u);
+                if (v.w != drop) atomicAdd(&s_h[wave][(v.w >> shift) & (RADIX - 1)], 1u);
             } else {
-                for (int64_t j = q * 4; j < end; ++j) atomicAdd(&s_h[wave][(keys[j] >> shift) & (RADIX - 1)], 1u);
+                for (int64_t j = q * 4; j < end; ++j)
+                    if (keys[j] != drop) atomicAdd(&s_h[wave][(keys[j] >> shift) & (RADIX - 1)], 1u);
             }
         }
     }
@@ -158,9 +164,12 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
                                                         uint32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out,
                                                         const uint32_t* __restrict__ hist, int64_t n_host,
                                                         const uint32_t* __restrict__ n_dev, int shift, int nblk,
-                                                        const int32_t* done) {
+                                                        const int32_t* done, uint32_t drop,
+                                                        uint32_t* __restrict__ n_out) {
     if (is_done(done)) return;
     constexpr int RADIX = 1 << BITS;
+    // The pass's total (pairs kept) for the passes after it.
+    if (n_out && blockIdx.x == 0 && threadIdx.x == 0) *n_out = hist[(int64_t)nblk << BITS];
     constexpr int DPT = RADIX > kThreads ? RADIX / kThreads : 1;  // digits per thread in the digit scan
     constexpr uint32_t MASK = RADIX - 1;
     __shared__ uint32_t s_keys[kTile];
@@ -169,7 +178,9 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
     __shared__ uint32_t s_start[RADIX];
     __shared__ uint32_t s_tot[RADIX];
     __shared__ uint32_t s_base[RADIX];
+    __shared__ uint32_t s_delta[RADIX];
     __shared__ uint32_t s_w[kWaves];
+    __shared__ uint32_t s_kept;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int64_t begin, end;
@@ -206,14 +217,21 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
 #pragma unroll
         for (int r = 0; r < kItems; ++r) {
             const int e = wave * (kItems * 64) + r * 64 + lane;
-            const bool valid = e < tile_n;
+            const bool valid = e < tile_n && k[r] != drop;
             const uint32_t d = (k[r] >> shift) & MASK;
+            // Lanes whose digit differs from this lane's in some bit: per
+            // bit, (ballot of the bit) XOR (this lane's bit, sign-extended).
             uint64_t peers = __ballot(valid);
+            {
+                uint32_t xlo = 0, xhi = 0;
 #pragma unroll
-            for (int b = 0; b < BITS; ++b) {
-                const bool bit = (d >> b) & 1u;
-                const uint64_t m = __ballot(bit);
-                peers &= bit ? m : ~m;
+                for (int b = 0; b < BITS; ++b) {
+                    const int32_t ext = ((int32_t)(d << (31 - b))) >> 31;
+                    const uint64_t m = __ballot(ext != 0);
+                    xlo |= (uint32_t)m ^ (uint32_t)ext;
+                    xhi |= (uint32_t)(m >> 32) ^ (uint32_t)ext;
+                }
+                peers &= ~(((uint64_t)xhi << 32) | xlo);
             }
             const uint32_t rk = (uint32_t)__popcll(peers & below);
             const uint32_t leader = peers ? (uint32_t)(__ffsll((unsigned long long)peers) - 1) : 0u;
@@ -246,11 +264,16 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
                 loc[j] = d < RADIX ? s_tot[d] : 0u;
                 sum += loc[j];
             }
-            uint32_t run = block_excl_scan(sum, s_w, nullptr);
+            uint32_t kept;
+            uint32_t run = block_excl_scan(sum, s_w, &kept);
+            if (tid == 0) s_kept = kept;
 #pragma unroll
             for (int j = 0; j < DPT; ++j) {
                 const int d = tid * DPT + j;
-                if (d < RADIX) s_start[d] = run;
+                if (d < RADIX) {
+                    s_start[d] = run;
+                    s_delta[d] = s_base[d] - run;  // staged pair i goes to s_delta[d] + i
+                }
                 run += loc[j];
             }
         }
@@ -258,7 +281,7 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
 #pragma unroll
         for (int r = 0; r < kItems; ++r) {
             const int e = wave * (kItems * 64) + r * 64 + lane;
-            if (e < tile_n) {
+            if (e < tile_n && k[r] != drop) {
                 const uint32_t d = (k[r] >> shift) & MASK;
                 const uint32_t pos = s_start[d] + s_wcnt[wave][d] + rank[r];
                 s_keys[pos] = k[r];
@@ -266,12 +289,25 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
             }
         }
         __syncthreads();
-        for (int i = tid; i < tile_n; i += kThreads) {
-            const uint32_t kk = s_keys[i];
-            const uint32_t d = (kk >> shift) & MASK;
-            const uint32_t g = s_base[d] + ((uint32_t)i - s_start[d]);
-            keys_out[g] = kk;
-            vals_out[g] = s_vals[i];
+        {
+            const int kept_n = (int)s_kept;
+            uint32_t kk[kItems], gg[kItems];
+            int32_t vv[kItems];
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) {
+                const int i = tid + j * kThreads;
+                kk[j] = i < kept_n ? s_keys[i] : 0u;
+                vv[j] = i < kept_n ? s_vals[i] : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) gg[j] = s_delta[(kk[j] >> shift) & MASK] + (uint32_t)(tid + j * kThreads);
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) {
+                if (tid + j * kThreads < kept_n) {
+                    keys_out[gg[j]] = kk[j];
+                    vals_out[gg[j]] = vv[j];
+                }
+            }
         }
         __syncthreads();
         for (int d = tid; d < RADIX; d += kThreads) s_base[d] += s_tot[d];
@@ -361,25 +397,27 @@ __global__ void k_offsets(const uint32_t* __restrict__ keys, int64_t n_host, con
 template <int BITS>
 hipError_t radix_pass(TableScratch& s, const uint32_t* kin, const int32_t* vin, int id_mode, int32_t N, uint32_t* kout,
                       int32_t* vout, int64_t n, const uint32_t* n_dev, int shift, int nblk, const int32_t* done,
-                      hipStream_t st, LaunchTimer* tm) {
+                      hipStream_t st, LaunchTimer* tm, uint32_t drop, uint32_t* n_out) {
     if (s.grouped_keys)
         launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, true>, dim3(nblk), dim3(kThreads), st, kin, n, n_dev, shift,
-               s.hist, nblk, done);
+               s.hist, nblk, done, drop);
     else
         launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, false>, dim3(nblk), dim3(kThreads), st, kin, n, n_dev, shift,
-               s.hist, nblk, done);
-    hipError_t e = exclusive_scan_u32(s.hist, s.hist, (int64_t)nblk << BITS, s.scan_part, done, st, tm, KC_SORT_SCAN);
+               s.hist, nblk, done, drop);
+    hipError_t e =
+        exclusive_scan_u32(s.hist, s.hist, ((int64_t)nblk << BITS) + 1, s.scan_part, done, st, tm, KC_SORT_SCAN);
     if (e != hipSuccess) return e;
     launch(tm, KC_SORT_DOWNSWEEP, k_downsweep<BITS>, dim3(nblk), dim3(kThreads), st, kin, vin, id_mode, N, kout, vout,
-           (const uint32_t*)s.hist, n, n_dev, shift, nblk, done);
+           (const uint32_t*)s.hist, n, n_dev, shift, nblk, done, drop, n_out);
     return hipGetLastError();
 }
 
 hipError_t radix_pass_bits(int bits, TableScratch& s, const uint32_t* kin, const int32_t* vin, int id_mode, int32_t N,
                            uint32_t* kout, int32_t* vout, int64_t n, const uint32_t* n_dev, int shift, int nblk,
-                           const int32_t* done, hipStream_t st, LaunchTimer* tm) {
+                           const int32_t* done, hipStream_t st, LaunchTimer* tm, uint32_t drop, uint32_t* n_out) {
 #define PT_PASS(B) \
-    case B: return radix_pass<B>(s, kin, vin, id_mode, N, kout, vout, n, n_dev, shift, nblk, done, st, tm);
+    case B:        \
+        return radix_pass<B>(s, kin, vin, id_mode, N, kout, vout, n, n_dev, shift, nblk, done, st, tm, drop, n_out);
     switch (bits) {
         PT_PASS(1) PT_PASS(2) PT_PASS(3) PT_PASS(4) PT_PASS(5) PT_PASS(6)
         PT_PASS(7) PT_PASS(8) PT_PASS(9) PT_PASS(10) PT_PASS(11)
@@ -403,11 +441,12 @@ int radix_passes(int32_t K) { return (key_bits(K) + g_max_bits - 1) / g_max_bits
 int table_scratch_alloc(TableScratch& s, int64_t n_cap) {
     table_scratch_free(s);
     const int64_t n = std::max<int64_t>(n_cap, 1);
-    const int64_t hist_n = ((int64_t)1 << kMaxBitsSupported) * kMaxSortBlocks;
+    const int64_t hist_n = ((int64_t)1 << kMaxBitsSupported) * kMaxSortBlocks + 1;
     if (hipMalloc(&s.keys_b, n * sizeof(uint32_t)) != hipSuccess) return -1;
     if (hipMalloc(&s.vals_a, n * sizeof(int32_t)) != hipSuccess) return -1;
     if (hipMalloc(&s.vals_b, n * sizeof(int32_t)) != hipSuccess) return -1;
     if (hipMalloc(&s.hist, (size_t)hist_n * sizeof(uint32_t)) != hipSuccess) return -1;
+    if (hipMalloc(&s.n_kept, sizeof(uint32_t)) != hipSuccess) return -1;
     // Partial sums for scans of up to max(n_cap, hist_n) values.
     const int64_t parts = (std::max<int64_t>(n, hist_n) + kScanSeg - 1) / kScanSeg + 2;
     if (hipMalloc(&s.scan_part, (size_t)parts * sizeof(uint32_t)) != hipSuccess) return -1;
@@ -420,6 +459,7 @@ void table_scratch_free(TableScratch& s) {
     if (s.vals_a) (void)hipFree(s.vals_a);
     if (s.vals_b) (void)hipFree(s.vals_b);
     if (s.hist) (void)hipFree(s.hist);
+    if (s.n_kept) (void)hipFree(s.n_kept);
     if (s.scan_part) (void)hipFree(s.scan_part);
     s = TableScratch();
 }
@@ -436,8 +476,10 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint
 
 hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int64_t n, const uint32_t* n_dev,
                        int32_t K, IdMode mode, int32_t N, int32_t* people_out, int32_t* offsets_out,
-                       const int32_t* done, hipStream_t stream, LaunchTimer* tm) {
+                       const int32_t* done, hipStream_t stream, LaunchTimer* tm, uint32_t drop_key) {
     if (n > s.n_cap || K < 1 || n >= (int64_t)1 << 32) return hipErrorInvalidValue;
+    if (drop_key != kNoDrop && (n_dev || drop_key < (uint32_t)K - 1u || drop_key >= (uint32_t)K))
+        return hipErrorInvalidValue;  // only the last bucket of a host-sized table can be dropped
     const int bits = key_bits(K);
     const int passes = radix_passes(K);
     // Split the bits as evenly as possible over the passes.
@@ -453,9 +495,13 @@ hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int
         int shift = 0;
         for (int p = 0; p < passes; ++p) {
             int32_t* vout = (p == passes - 1) ? people_out : ((p & 1) ? s.vals_b : s.vals_a);
+            const bool dropping = p == 0 && drop_key != kNoDrop;
             hipError_t e = radix_pass_bits(pass_bits[p], s, kin, vin, mode, N, kout, vout, n, n_dev, shift, nblk, done,
-                                           stream, tm);
+                                           stream, tm, dropping ? drop_key : kNoDrop, dropping ? s.n_kept : nullptr);
             if (e != hipSuccess) return e;
+            // After a first pass that dropped keys the remaining passes
+            // sort the pairs it kept (count on the device).
+            if (dropping) n_dev = s.n_kept;
             shift += pass_bits[p];
             vin = vout;
             std::swap(kin, kout);
