@@ -86,10 +86,22 @@ struct U4 { uint32_t w[4]; };
 __host__ __device__ __forceinline__ uint32_t word(const U4& u, int i) {
     return i == 0 ? u.w[0] : (i == 1 ? u.w[1] : (i == 2 ? u.w[2] : u.w[3]));
 }
+// 64-bit rotate left by a constant 0 < n < 64 as two 32-bit funnel shifts
+// (v_alignbit_b32): hi' = (hi:lo) >> (32 - n), lo' = (lo:hi) >> (32 - n)
+// for n < 32; halves swapped first for n >= 32.
+template <unsigned n>
+__device__ __forceinline__ uint64_t rotl64c(uint64_t x) {
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    if (n == 32) return ((uint64_t)lo << 32) | hi;
+    const uint32_t a = n < 32 ? hi : lo, b = n < 32 ? lo : hi;
+    constexpr unsigned s = 32u - (n & 31u);
+    return ((uint64_t)__builtin_amdgcn_alignbit(a, b, s) << 32) | __builtin_amdgcn_alignbit(b, a, s);
+}
+
 __device__ __forceinline__ U4 draw4(uint64_t c, uint64_t k0, uint64_t k1) {
     const uint64_t k2 = 0x1BD11BDAA9FC1A22ull ^ k0 ^ k1;
     uint64_t x0 = c + k0, x1 = c + k1;
-#define PT_R(n) x0 += x1; x1 = rotl64(x1, n); x1 ^= x0;
+#define PT_R(n) x0 += x1; x1 = rotl64c<n>(x1); x1 ^= x0;
 #define PT_INJ(a, b, s) x0 += a; x1 += b + (uint64_t)(s);
     PT_R(16) PT_R(42) PT_R(12) PT_R(31) PT_INJ(k1, k2, 1)
     PT_R(16) PT_R(32) PT_R(24) PT_R(21) PT_INJ(k2, k0, 2)
