@@ -74,9 +74,9 @@ enum IdMode : int {
 };
 
 struct TableScratch {
-    uint32_t* keys_b = nullptr;  // n_cap
-    int32_t* vals_a = nullptr;   // n_cap
-    int32_t* vals_b = nullptr;   // n_cap
+    uint32_t* keys_b = nullptr;  // n_cap: the sorted keys of the last pass
+    uint2* pairs_a = nullptr;    // n_cap (key, id) pairs between passes
+    uint2* pairs_b = nullptr;    // n_cap
     uint32_t* hist = nullptr;    // 256 * kMaxSortBlocks
     uint32_t* scan_part = nullptr;
     uint32_t* n_kept = nullptr;  // pairs kept by a first pass that dropped a key (device)
@@ -91,7 +91,7 @@ void table_scratch_free(TableScratch& s);
 
 constexpr uint32_t kNoDrop = 0xFFFFFFFFu;
 
-// Builds the table.  keys is clobbered.  vals (may be null) holds explicit
+// Builds the table.  keys is only read.  vals (may be null) holds explicit
 // ids in input order; when null the ids follow from mode.  When n_dev is
 // set it holds the pair count on the device and n is only the capacity.
 // done (device int, may be null) turns every kernel into a no-op when set.

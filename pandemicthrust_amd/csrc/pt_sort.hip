@@ -5,6 +5,13 @@
 // (PandemicSim.cu:1212-1248, :1316-1354, :3835-3862).  Keys are dense bucket
 // ids, so only ceil(log2 K) bits are sorted (3 passes of <= 8 bits at 50M
 // agents instead of the reference's 7 passes over all 32 bits).
+//
+// Data between passes: the first pass reads the caller's keys (ids explicit
+// or implied by the position) and writes interleaved (key, id) pairs, one
+// 8-byte store per pair; middle passes read and write pairs; the last pass
+// writes keys and ids to separate arrays (ids are the table, keys feed the
+// bucket offsets).  Full 8-bit passes run 1024-thread blocks (8192-pair
+// tiles, so digit runs average 32 pairs), narrower digits 512-thread blocks.
 #include "pt_sort.h"
 
 #include <algorithm>
@@ -13,23 +20,26 @@ namespace pt {
 
 namespace {
 
-#ifndef PT_SORT_THREADS
-#define PT_SORT_THREADS 512
-#endif
-constexpr int kThreads = PT_SORT_THREADS;
-constexpr int kWaves = kThreads / 64;
 #ifndef PT_SORT_ITEMS
 #define PT_SORT_ITEMS 8
 #endif
 constexpr int kItems = PT_SORT_ITEMS;  // pairs per thread per tile
-constexpr int kTile = kThreads * kItems;  // pairs per tile
-#ifndef PT_SORT_BLOCKS
-#define PT_SORT_BLOCKS 1024
-#endif
-constexpr int kMaxSortBlocks = PT_SORT_BLOCKS;
-constexpr int kMaxBitsSupported = 11;
+constexpr int kScanThreads = 512;
 constexpr int kScanSeg = 8192;  // elements per scan block
-constexpr int kScanItems = kScanSeg / kThreads;
+constexpr int kScanItems = kScanSeg / kScanThreads;
+constexpr int kMaxBitsSupported = 11;
+constexpr int kHistBlocks = 1024;  // most blocks of any pass
+
+// Blocks of a pass with T threads: 1024 x 512 threads or 256 x 1024.
+template <int T>
+constexpr int max_blocks() {
+    return T >= 1024 ? 256 : 1024;
+}
+// Block width of a pass with BITS-bit digits.
+template <int BITS>
+constexpr int pass_threads() {
+    return BITS == 8 ? 1024 : 512;
+}
 
 int g_max_bits = 8;  // widest digit per pass (see set_radix_max_bits)
 
@@ -41,13 +51,16 @@ __device__ __forceinline__ int32_t implicit_id(int64_t j, int mode, int32_t N) {
 
 __device__ __forceinline__ bool is_done(const int32_t* done) { return done && *done; }
 
-// The block's slice of the n pairs (n from the device when n_dev is set).
+// The block's slice of the n pairs (n from the device when n_dev is set),
+// whole tiles of T * kItems pairs.
+template <int T>
 __device__ __forceinline__ void block_range(int64_t n_host, const uint32_t* n_dev, int64_t& begin, int64_t& end) {
+    constexpr int64_t tile = (int64_t)T * kItems;
     const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
-    const int64_t tiles = (n + kTile - 1) / kTile;
+    const int64_t tiles = (n + tile - 1) / tile;
     const int64_t tpb = (tiles + gridDim.x - 1) / gridDim.x;
-    begin = (int64_t)blockIdx.x * tpb * kTile;
-    end = min(n, begin + tpb * kTile);
+    begin = (int64_t)blockIdx.x * tpb * tile;
+    end = min(n, begin + tpb * tile);
 }
 
 // Inclusive scan across a wave64.
@@ -60,15 +73,18 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
     return v;
 }
 
-// Exclusive scan of one value per thread over the block (s_w: kWaves words).
+// Exclusive scan of one value per thread over a block of T threads (s_w:
+// T / 64 words).
+template <int T>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, uint32_t* total) {
+    constexpr int W = T / 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t inc = wave_incl_scan(v, lane);
     if (lane == 63) s_w[wave] = inc;
     __syncthreads();
     uint32_t wbase = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
+    for (int w = 0; w < W; ++w) {
         const uint32_t x = s_w[w];
         if (w < wave) wbase += x;
         tot += x;
@@ -78,140 +94,160 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, u
     return wbase + inc - v;
 }
 
-// Per-block digit histogram of the block's chunk.
-// Keys equal to drop are left out (not counted, not written by the
-// matching downsweep).  hist[nblk << BITS] is set to 0 so that the scan of
-// nblk * RADIX + 1 entries leaves the pass's total there.
-template <int BITS, bool MATCH>
-__global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t* __restrict__ keys, int64_t n_host,
-                                                      const uint32_t* __restrict__ n_dev, int shift,
-                                                      uint32_t* __restrict__ hist, int nblk, const int32_t* done,
-                                                      uint32_t drop) {
+// Per-block digit histogram of the block's chunk, from a key array (PIN
+// false, 4 keys per 16-byte load) or from interleaved pairs (PIN true, 2
+// pairs per load).  Keys equal to drop are left out (not counted, not
+// written by the matching downsweep).  hist[nblk << BITS] is set to 0 so
+// that the scan of nblk * RADIX + 1 entries leaves the pass's total there.
+template <int BITS, bool MATCH, bool PIN, int T>
+__global__ __launch_bounds__(T) void k_upsweep(const uint32_t* __restrict__ keys, const uint2* __restrict__ pairs,
+                                               int64_t n_host, const uint32_t* __restrict__ n_dev, int shift,
+                                               uint32_t* __restrict__ hist, int nblk, const int32_t* done,
+                                               uint32_t drop) {
     if (is_done(done)) return;
     constexpr int RADIX = 1 << BITS;
+    constexpr int W = T / 64;
+    constexpr int G = PIN ? 2 : 4;  // keys per 16-byte load
     if (blockIdx.x == 0 && threadIdx.x == 0) hist[(int64_t)nblk << BITS] = 0;
-    __shared__ uint32_t s_h[kWaves][RADIX];
+    __shared__ uint32_t s_h[W][RADIX];
     const int tid = threadIdx.x, wave = tid >> 6;
-    for (int i = tid; i < kWaves * RADIX; i += kThreads) (&s_h[0][0])[i] = 0;
+    for (int i = tid; i < W * RADIX; i += T) (&s_h[0][0])[i] = 0;
     __syncthreads();
     int64_t begin, end;
-    block_range(n_host, n_dev, begin, end);
-    const uint4* k4 = reinterpret_cast<const uint4*>(keys);
-    if (MATCH) {
-        // Keys with long runs of one digit (e.g. grouped by simulation): when
-        // every active lane of the wave holds the same digit, one lane adds
-        // the count; otherwise each lane adds its own.
-        for (int64_t q = begin / 4 + tid; q * 4 < end; q += kThreads) {
-            uint32_t kk[4];
-            if (q * 4 + 3 < end) {
-                const uint4 v = k4[q];
-                kk[0] = v.x;
-                kk[1] = v.y;
-                kk[2] = v.z;
-                kk[3] = v.w;
+    block_range<T>(n_host, n_dev, begin, end);
+    const uint4* v4 = PIN ? reinterpret_cast<const uint4*>(pairs) : reinterpret_cast<const uint4*>(keys);
+    // U 16-byte loads in flight per thread and iteration.
+    constexpr int U = 4;
+    for (int64_t q0 = begin / G + tid; q0 * G < end; q0 += (int64_t)U * T) {
+        uint32_t kk[U][G];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t q = q0 + (int64_t)u * T;
+            if (q * G + G - 1 < end) {
+                const uint4 v = v4[q];
+                if (PIN) {
+                    kk[u][0] = v.x;
+                    kk[u][G - 1] = v.z;
+                } else {
+                    kk[u][0] = v.x;
+                    kk[u][1 % G] = v.y;
+                    kk[u][2 % G] = v.z;
+                    kk[u][3 % G] = v.w;
+                }
             } else {
 #pragma unroll
-                for (int c = 0; c < 4; ++c) kk[c] = q * 4 + c < end ? keys[q * 4 + c] : 0u;
+                for (int c = 0; c < G; ++c) {
+                    const int64_t j = q * G + c;
+                    kk[u][c] = j < end ? (PIN ? pairs[j].x : keys[j]) : 0u;
+                }
             }
+        }
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const bool valid = q * 4 + c < end && kk[c] != drop;
-                const uint32_t d = (kk[c] >> shift) & (RADIX - 1);
-                const uint64_t act = __ballot(valid);
-                const int first = act ? __ffsll((unsigned long long)act) - 1 : 0;  // lowest valid lane
-                const uint32_t d0 = (uint32_t)__shfl((int)d, first, 64);
-                const uint64_t same = __ballot(valid && d == d0);
-                if (same == act) {
-                    if (valid && (threadIdx.x & 63) == first) atomicAdd(&s_h[wave][d0], (uint32_t)__popcll(act));
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int c = 0; c < G; ++c) {
+                const bool valid = (q0 + (int64_t)u * T) * G + c < end && kk[u][c] != drop;
+                const uint32_t d = (kk[u][c] >> shift) & (RADIX - 1);
+                if (MATCH) {
+                    // Keys with long runs of one digit (e.g. grouped by
+                    // simulation): when every active lane of the wave holds
+                    // the same digit, one lane adds the count.
+                    const uint64_t act = __ballot(valid);
+                    const int first = act ? __ffsll((unsigned long long)act) - 1 : 0;  // lowest valid lane
+                    const uint32_t d0 = (uint32_t)__shfl((int)d, first, 64);
+                    const uint64_t same = __ballot(valid && d == d0);
+                    if (same == act) {
+                        if (valid && (threadIdx.x & 63) == first)
+                            atomicAdd(&s_h[wave][d0], (uint32_t)__popcll(act));
+                    } else if (valid) {
+                        atomicAdd(&s_h[wave][d], 1u);
+                    }
                 } else if (valid) {
                     atomicAdd(&s_h[wave][d], 1u);
                 }
             }
         }
-    } else {
-        for (int64_t q = begin / 4 + tid; q * 4 < end; q += kThreads) {
-            if (q * 4 + 3 < end) {
-                const uint4 v = k4[q];
-                if (v.x != drop) atomicAdd(&s_h[wave][(v.x >> shift) & (RADIX - 1)], 1u);
-                if (v.y != drop) atomicAdd(&s_h[wave][(v.y >> shift) & (RADIX - 1)], 1u);
-                if (v.z != drop) atomicAdd(&s_h[wave][(v.z >> shift) & (RADIX - 1)], 1u);
-                if (v.w != drop) atomicAdd(&s_h[wave][(v.w >> shift) & (RADIX - 1)], 1u);
-            } else {
-                for (int64_t j = q * 4; j < end; ++j)
-                    if (keys[j] != drop) atomicAdd(&s_h[wave][(keys[j] >> shift) & (RADIX - 1)], 1u);
-            }
-        }
     }
     __syncthreads();
-    for (int d = tid; d < RADIX; d += kThreads) {
+    for (int d = tid; d < RADIX; d += T) {
         uint32_t t = 0;
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) t += s_h[w][d];
+        for (int w = 0; w < W; ++w) t += s_h[w][d];
         hist[(int64_t)d * nblk + blockIdx.x] = t;
     }
 }
 
-// Stable scatter of the block's chunk by one digit.  Each tile of 4096 pairs
-// is ranked per wave with ballots (match-any over the digit bits): the lowest
-// lane of each digit group adds the group's size to the wave's LDS counter
-// with a returning atomic, so the 8 rounds of a wave issue back to back and
-// LDS order gives each round the count of the rounds before it.  The tile is
-// then staged in LDS in digit order and written out as runs per digit, while
-// the next tile's pairs are already in flight.
-template <int BITS>
-__global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restrict__ keys_in,
-                                                        const int32_t* __restrict__ vals_in, int id_mode, int32_t N,
-                                                        uint32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out,
-                                                        const uint32_t* __restrict__ hist, int64_t n_host,
-                                                        const uint32_t* __restrict__ n_dev, int shift, int nblk,
-                                                        const int32_t* done, uint32_t drop,
-                                                        uint32_t* __restrict__ n_out) {
+// Stable scatter of the block's chunk by one digit.  Each tile of T * 8
+// pairs is ranked per wave with ballots (match-any over the digit bits): the
+// lowest lane of each digit group adds the group's size to the wave's LDS
+// counter with a returning atomic, so the 8 rounds of a wave issue back to
+// back and LDS order gives each round the count of the rounds before it.
+// The tile is then staged in LDS in digit order and written out as runs per
+// digit, while the next tile's pairs are already in flight.  Input: keys
+// plus explicit or implied ids (PIN false) or pairs (PIN true); output:
+// pairs (POUT true) or separate keys and ids.
+template <int BITS, bool PIN, bool POUT, int T>
+__global__ __launch_bounds__(T) void k_downsweep(const uint32_t* __restrict__ keys_in,
+                                                 const int32_t* __restrict__ vals_in, const uint2* __restrict__ pairs_in,
+                                                 int id_mode, int32_t N, uint32_t* __restrict__ keys_out,
+                                                 int32_t* __restrict__ vals_out, uint2* __restrict__ pairs_out,
+                                                 const uint32_t* __restrict__ hist, int64_t n_host,
+                                                 const uint32_t* __restrict__ n_dev, int shift, int nblk,
+                                                 const int32_t* done, uint32_t drop, uint32_t* __restrict__ n_out) {
     if (is_done(done)) return;
     constexpr int RADIX = 1 << BITS;
+    constexpr int W = T / 64;
+    constexpr int TILE = T * kItems;
     // The pass's total (pairs kept) for the passes after it.
     if (n_out && blockIdx.x == 0 && threadIdx.x == 0) *n_out = hist[(int64_t)nblk << BITS];
-    constexpr int DPT = RADIX > kThreads ? RADIX / kThreads : 1;  // digits per thread in the digit scan
+    constexpr int DPT = RADIX > T ? RADIX / T : 1;  // digits per thread in the digit scan
     constexpr uint32_t MASK = RADIX - 1;
-    __shared__ uint32_t s_keys[kTile];
-    __shared__ int32_t s_vals[kTile];
-    __shared__ uint32_t s_wcnt[kWaves][RADIX];
+    __shared__ uint2 s_pairs[TILE];
+    __shared__ uint32_t s_wcnt[W][RADIX];
     __shared__ uint32_t s_start[RADIX];
     __shared__ uint32_t s_tot[RADIX];
     __shared__ uint32_t s_base[RADIX];
     __shared__ uint32_t s_delta[RADIX];
-    __shared__ uint32_t s_w[kWaves];
+    __shared__ uint32_t s_w[W];
     __shared__ uint32_t s_kept;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int64_t begin, end;
-    block_range(n_host, n_dev, begin, end);
-    for (int d = tid; d < RADIX; d += kThreads) s_base[d] = hist[(int64_t)d * nblk + blockIdx.x];
+    block_range<T>(n_host, n_dev, begin, end);
+    for (int d = tid; d < RADIX; d += T) s_base[d] = hist[(int64_t)d * nblk + blockIdx.x];
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+    auto load = [&](int64_t j, bool valid, uint32_t& k, int32_t& v) {
+        if (PIN) {
+            const uint2 p = valid ? pairs_in[j] : make_uint2(0u, 0u);
+            k = p.x;
+            v = (int32_t)p.y;
+        } else {
+            k = valid ? keys_in[j] : 0u;
+            v = vals_in ? (valid ? vals_in[j] : 0) : implicit_id(j, id_mode, N);
+        }
+    };
 
     uint32_t k[kItems];
     int32_t v[kItems];
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
         const int64_t j = begin + wave * (kItems * 64) + r * 64 + lane;
-        const bool valid = j < end;
-        k[r] = valid ? keys_in[j] : 0u;
-        v[r] = vals_in ? (valid ? vals_in[j] : 0) : implicit_id(j, id_mode, N);
+        load(j, j < end, k[r], v[r]);
     }
 
-    for (int64_t t0 = begin; t0 < end; t0 += kTile) {
-        const int tile_n = (int)min((int64_t)kTile, end - t0);
-        for (int i = tid; i < kWaves * RADIX; i += kThreads) (&s_wcnt[0][0])[i] = 0;
+    for (int64_t t0 = begin; t0 < end; t0 += TILE) {
+        const int tile_n = (int)min((int64_t)TILE, end - t0);
+        for (int i = tid; i < W * RADIX; i += T) (&s_wcnt[0][0])[i] = 0;
         __syncthreads();
         // Next tile in flight while this one is ranked and written.
         uint32_t nk[kItems];
         int32_t nv[kItems];
-        const int64_t t1 = t0 + kTile;
+        const int64_t t1 = t0 + TILE;
 #pragma unroll
         for (int r = 0; r < kItems; ++r) {
             const int64_t j = t1 + wave * (kItems * 64) + r * 64 + lane;
-            const bool valid = j < end;
-            nk[r] = valid ? keys_in[j] : 0u;
-            nv[r] = vals_in ? (valid ? vals_in[j] : 0) : implicit_id(j, id_mode, N);
+            load(j, j < end, nk[r], nv[r]);
         }
         uint32_t old[kItems], info[kItems];
 #pragma unroll
@@ -244,10 +280,10 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
 #pragma unroll
         for (int r = 0; r < kItems; ++r) rank[r] = __shfl(old[r], (int)(info[r] >> 8), 64) + (info[r] & 0xFF);
         __syncthreads();
-        for (int d = tid; d < RADIX; d += kThreads) {
+        for (int d = tid; d < RADIX; d += T) {
             uint32_t run = 0;
 #pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
+            for (int w = 0; w < W; ++w) {
                 const uint32_t c = s_wcnt[w][d];
                 s_wcnt[w][d] = run;
                 run += c;
@@ -265,7 +301,7 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
                 sum += loc[j];
             }
             uint32_t kept;
-            uint32_t run = block_excl_scan(sum, s_w, &kept);
+            uint32_t run = block_excl_scan<T>(sum, s_w, &kept);
             if (tid == 0) s_kept = kept;
 #pragma unroll
             for (int j = 0; j < DPT; ++j) {
@@ -284,33 +320,35 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
             if (e < tile_n && k[r] != drop) {
                 const uint32_t d = (k[r] >> shift) & MASK;
                 const uint32_t pos = s_start[d] + s_wcnt[wave][d] + rank[r];
-                s_keys[pos] = k[r];
-                s_vals[pos] = v[r];
+                s_pairs[pos] = make_uint2(k[r], (uint32_t)v[r]);
             }
         }
         __syncthreads();
         {
             const int kept_n = (int)s_kept;
-            uint32_t kk[kItems], gg[kItems];
-            int32_t vv[kItems];
+            uint2 pp[kItems];
+            uint32_t gg[kItems];
 #pragma unroll
             for (int j = 0; j < kItems; ++j) {
-                const int i = tid + j * kThreads;
-                kk[j] = i < kept_n ? s_keys[i] : 0u;
-                vv[j] = i < kept_n ? s_vals[i] : 0;
+                const int i = tid + j * T;
+                pp[j] = i < kept_n ? s_pairs[i] : make_uint2(0u, 0u);
             }
 #pragma unroll
-            for (int j = 0; j < kItems; ++j) gg[j] = s_delta[(kk[j] >> shift) & MASK] + (uint32_t)(tid + j * kThreads);
+            for (int j = 0; j < kItems; ++j) gg[j] = s_delta[(pp[j].x >> shift) & MASK] + (uint32_t)(tid + j * T);
 #pragma unroll
             for (int j = 0; j < kItems; ++j) {
-                if (tid + j * kThreads < kept_n) {
-                    keys_out[gg[j]] = kk[j];
-                    vals_out[gg[j]] = vv[j];
+                if (tid + j * T < kept_n) {
+                    if (POUT) {
+                        pairs_out[gg[j]] = pp[j];
+                    } else {
+                        keys_out[gg[j]] = pp[j].x;
+                        vals_out[gg[j]] = (int32_t)pp[j].y;
+                    }
                 }
             }
         }
         __syncthreads();
-        for (int d = tid; d < RADIX; d += kThreads) s_base[d] += s_tot[d];
+        for (int d = tid; d < RADIX; d += T) s_base[d] += s_tot[d];
 #pragma unroll
         for (int r = 0; r < kItems; ++r) {
             k[r] = nk[r];
@@ -320,42 +358,42 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t* __restri
 }
 
 // Scan, phase 1: per-segment sums.
-__global__ __launch_bounds__(kThreads) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
-                                                          uint32_t* __restrict__ part, const int32_t* done) {
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
+                                                              uint32_t* __restrict__ part, const int32_t* done) {
     if (is_done(done)) return;
-    __shared__ uint32_t s_w[kWaves];
+    __shared__ uint32_t s_w[kScanThreads / 64];
     const int64_t base = (int64_t)blockIdx.x * kScanSeg;
     uint32_t sum = 0;
-    for (int i = threadIdx.x; i < kScanSeg; i += kThreads) {
+    for (int i = threadIdx.x; i < kScanSeg; i += kScanThreads) {
         const int64_t j = base + i;
         if (j < n) sum += in[j];
     }
     uint32_t tot;
-    block_excl_scan(sum, s_w, &tot);
+    block_excl_scan<kScanThreads>(sum, s_w, &tot);
     if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
 // Scan, phase 2: exclusive scan of the segment sums (one block).
-__global__ __launch_bounds__(kThreads) void k_scan_parts(uint32_t* part, int nparts, const int32_t* done) {
+__global__ __launch_bounds__(kScanThreads) void k_scan_parts(uint32_t* part, int nparts, const int32_t* done) {
     if (is_done(done)) return;
-    __shared__ uint32_t s_w[kWaves];
+    __shared__ uint32_t s_w[kScanThreads / 64];
     uint32_t carry = 0;
-    for (int b0 = 0; b0 < nparts; b0 += kThreads) {
+    for (int b0 = 0; b0 < nparts; b0 += kScanThreads) {
         const int i = b0 + threadIdx.x;
         const uint32_t v = i < nparts ? part[i] : 0u;
         uint32_t tot;
-        const uint32_t ex = block_excl_scan(v, s_w, &tot);
+        const uint32_t ex = block_excl_scan<kScanThreads>(v, s_w, &tot);
         if (i < nparts) part[i] = carry + ex;
         carry += tot;
     }
 }
 
 // Scan, phase 3: exclusive scan inside each segment plus its offset.
-__global__ __launch_bounds__(kThreads) void k_scan_down(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                        int64_t n, const uint32_t* __restrict__ part,
-                                                        const int32_t* done) {
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t* __restrict__ in,
+                                                            uint32_t* __restrict__ out, int64_t n,
+                                                            const uint32_t* __restrict__ part, const int32_t* done) {
     if (is_done(done)) return;
-    __shared__ uint32_t s_w[kWaves];
+    __shared__ uint32_t s_w[kScanThreads / 64];
     const int64_t base = (int64_t)blockIdx.x * kScanSeg + (int64_t)threadIdx.x * kScanItems;
     uint32_t x[kScanItems];
     uint32_t sum = 0;
@@ -365,7 +403,7 @@ __global__ __launch_bounds__(kThreads) void k_scan_down(const uint32_t* __restri
         x[i] = j < n ? in[j] : 0u;
         sum += x[i];
     }
-    uint32_t run = block_excl_scan(sum, s_w, nullptr) + part[blockIdx.x];
+    uint32_t run = block_excl_scan<kScanThreads>(sum, s_w, nullptr) + part[blockIdx.x];
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
         const int64_t j = base + i;
@@ -394,30 +432,61 @@ __global__ void k_offsets(const uint32_t* __restrict__ keys, int64_t n_host, con
     }
 }
 
+// One radix pass.  Input: keys + vals (or implied ids) when pin is null,
+// else pairs pin; output: pairs pout when set, else keys kout + ids vout.
+struct PassIO {
+    const uint32_t* kin;
+    const int32_t* vin;
+    const uint2* pin;
+    uint32_t* kout;
+    int32_t* vout;
+    uint2* pout;
+};
+
 template <int BITS>
-hipError_t radix_pass(TableScratch& s, const uint32_t* kin, const int32_t* vin, int id_mode, int32_t N, uint32_t* kout,
-                      int32_t* vout, int64_t n, const uint32_t* n_dev, int shift, int nblk, const int32_t* done,
-                      hipStream_t st, LaunchTimer* tm, uint32_t drop, uint32_t* n_out) {
-    if (s.grouped_keys)
-        launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, true>, dim3(nblk), dim3(kThreads), st, kin, n, n_dev, shift,
-               s.hist, nblk, done, drop);
+hipError_t radix_pass(TableScratch& s, const PassIO& io, int id_mode, int32_t N, int64_t n, const uint32_t* n_dev,
+                      int shift, const int32_t* done, hipStream_t st, LaunchTimer* tm, uint32_t drop,
+                      uint32_t* n_out) {
+    constexpr int T = pass_threads<BITS>();
+    const int64_t tiles = (n + (int64_t)T * kItems - 1) / ((int64_t)T * kItems);
+    const int nblk = (int)std::min<int64_t>(std::max<int64_t>(tiles, 1), max_blocks<T>());
+    const bool pin = io.pin != nullptr;
+    if (pin && s.grouped_keys)
+        launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, true, true, T>, dim3(nblk), dim3(T), st, io.kin, io.pin, n, n_dev,
+               shift, s.hist, nblk, done, drop);
+    else if (pin)
+        launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, false, true, T>, dim3(nblk), dim3(T), st, io.kin, io.pin, n, n_dev,
+               shift, s.hist, nblk, done, drop);
+    else if (s.grouped_keys)
+        launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, true, false, T>, dim3(nblk), dim3(T), st, io.kin, io.pin, n, n_dev,
+               shift, s.hist, nblk, done, drop);
     else
-        launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, false>, dim3(nblk), dim3(kThreads), st, kin, n, n_dev, shift,
-               s.hist, nblk, done, drop);
+        launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, false, false, T>, dim3(nblk), dim3(T), st, io.kin, io.pin, n,
+               n_dev, shift, s.hist, nblk, done, drop);
     hipError_t e =
         exclusive_scan_u32(s.hist, s.hist, ((int64_t)nblk << BITS) + 1, s.scan_part, done, st, tm, KC_SORT_SCAN);
     if (e != hipSuccess) return e;
-    launch(tm, KC_SORT_DOWNSWEEP, k_downsweep<BITS>, dim3(nblk), dim3(kThreads), st, kin, vin, id_mode, N, kout, vout,
-           (const uint32_t*)s.hist, n, n_dev, shift, nblk, done, drop, n_out);
+    const uint32_t* h = s.hist;
+#define PT_DOWN(PI, PO)                                                                                              \
+    launch(tm, KC_SORT_DOWNSWEEP, k_downsweep<BITS, PI, PO, T>, dim3(nblk), dim3(T), st, io.kin, io.vin, io.pin,      \
+           id_mode, N, io.kout, io.vout, io.pout, h, n, n_dev, shift, nblk, done, drop, n_out)
+    if (pin && io.pout)
+        PT_DOWN(true, true);
+    else if (pin)
+        PT_DOWN(true, false);
+    else if (io.pout)
+        PT_DOWN(false, true);
+    else
+        PT_DOWN(false, false);
+#undef PT_DOWN
     return hipGetLastError();
 }
 
-hipError_t radix_pass_bits(int bits, TableScratch& s, const uint32_t* kin, const int32_t* vin, int id_mode, int32_t N,
-                           uint32_t* kout, int32_t* vout, int64_t n, const uint32_t* n_dev, int shift, int nblk,
-                           const int32_t* done, hipStream_t st, LaunchTimer* tm, uint32_t drop, uint32_t* n_out) {
+hipError_t radix_pass_bits(int bits, TableScratch& s, const PassIO& io, int id_mode, int32_t N, int64_t n,
+                           const uint32_t* n_dev, int shift, const int32_t* done, hipStream_t st, LaunchTimer* tm,
+                           uint32_t drop, uint32_t* n_out) {
 #define PT_PASS(B) \
-    case B:        \
-        return radix_pass<B>(s, kin, vin, id_mode, N, kout, vout, n, n_dev, shift, nblk, done, st, tm, drop, n_out);
+    case B: return radix_pass<B>(s, io, id_mode, N, n, n_dev, shift, done, st, tm, drop, n_out);
     switch (bits) {
         PT_PASS(1) PT_PASS(2) PT_PASS(3) PT_PASS(4) PT_PASS(5) PT_PASS(6)
         PT_PASS(7) PT_PASS(8) PT_PASS(9) PT_PASS(10) PT_PASS(11)
@@ -441,10 +510,10 @@ int radix_passes(int32_t K) { return (key_bits(K) + g_max_bits - 1) / g_max_bits
 int table_scratch_alloc(TableScratch& s, int64_t n_cap) {
     table_scratch_free(s);
     const int64_t n = std::max<int64_t>(n_cap, 1);
-    const int64_t hist_n = ((int64_t)1 << kMaxBitsSupported) * kMaxSortBlocks + 1;
+    const int64_t hist_n = ((int64_t)1 << kMaxBitsSupported) * kHistBlocks + 1;
     if (hipMalloc(&s.keys_b, n * sizeof(uint32_t)) != hipSuccess) return -1;
-    if (hipMalloc(&s.vals_a, n * sizeof(int32_t)) != hipSuccess) return -1;
-    if (hipMalloc(&s.vals_b, n * sizeof(int32_t)) != hipSuccess) return -1;
+    if (hipMalloc(&s.pairs_a, n * sizeof(uint2)) != hipSuccess) return -1;
+    if (hipMalloc(&s.pairs_b, n * sizeof(uint2)) != hipSuccess) return -1;
     if (hipMalloc(&s.hist, (size_t)hist_n * sizeof(uint32_t)) != hipSuccess) return -1;
     if (hipMalloc(&s.n_kept, sizeof(uint32_t)) != hipSuccess) return -1;
     // Partial sums for scans of up to max(n_cap, hist_n) values.
@@ -456,8 +525,8 @@ int table_scratch_alloc(TableScratch& s, int64_t n_cap) {
 
 void table_scratch_free(TableScratch& s) {
     if (s.keys_b) (void)hipFree(s.keys_b);
-    if (s.vals_a) (void)hipFree(s.vals_a);
-    if (s.vals_b) (void)hipFree(s.vals_b);
+    if (s.pairs_a) (void)hipFree(s.pairs_a);
+    if (s.pairs_b) (void)hipFree(s.pairs_b);
     if (s.hist) (void)hipFree(s.hist);
     if (s.n_kept) (void)hipFree(s.n_kept);
     if (s.scan_part) (void)hipFree(s.scan_part);
@@ -468,9 +537,9 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint
                               hipStream_t stream, LaunchTimer* tm, int cls) {
     if (n <= 0) return hipSuccess;
     const int nparts = (int)((n + kScanSeg - 1) / kScanSeg);
-    launch(tm, cls, k_scan_reduce, dim3(nparts), dim3(kThreads), stream, in, n, part, done);
-    launch(tm, cls, k_scan_parts, dim3(1), dim3(kThreads), stream, part, nparts, done);
-    launch(tm, cls, k_scan_down, dim3(nparts), dim3(kThreads), stream, in, out, n, (const uint32_t*)part, done);
+    launch(tm, cls, k_scan_reduce, dim3(nparts), dim3(kScanThreads), stream, in, n, part, done);
+    launch(tm, cls, k_scan_parts, dim3(1), dim3(kScanThreads), stream, part, nparts, done);
+    launch(tm, cls, k_scan_down, dim3(nparts), dim3(kScanThreads), stream, in, out, n, (const uint32_t*)part, done);
     return hipGetLastError();
 }
 
@@ -485,31 +554,37 @@ hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int
     // Split the bits as evenly as possible over the passes.
     int pass_bits[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int p = 0; p < passes; ++p) pass_bits[p] = bits / passes + (p < bits % passes ? 1 : 0);
-    uint32_t* kin = keys;
     if (n > 0) {
         // n is the capacity when n_dev carries the actual count.
-        const int64_t tiles = (n + kTile - 1) / kTile;
-        const int nblk = (int)std::min<int64_t>(tiles, kMaxSortBlocks);
-        uint32_t* kout = s.keys_b;
-        const int32_t* vin = vals;
         int shift = 0;
         for (int p = 0; p < passes; ++p) {
-            int32_t* vout = (p == passes - 1) ? people_out : ((p & 1) ? s.vals_b : s.vals_a);
+            const bool last = p == passes - 1;
+            PassIO io{};
+            if (p == 0) {
+                io.kin = keys;
+                io.vin = vals;
+            } else {
+                io.pin = (p & 1) ? s.pairs_a : s.pairs_b;
+            }
+            if (last) {
+                io.kout = s.keys_b;
+                io.vout = people_out;
+            } else {
+                io.pout = (p & 1) ? s.pairs_b : s.pairs_a;
+            }
             const bool dropping = p == 0 && drop_key != kNoDrop;
-            hipError_t e = radix_pass_bits(pass_bits[p], s, kin, vin, mode, N, kout, vout, n, n_dev, shift, nblk, done,
-                                           stream, tm, dropping ? drop_key : kNoDrop, dropping ? s.n_kept : nullptr);
+            hipError_t e = radix_pass_bits(pass_bits[p], s, io, mode, N, n, n_dev, shift, done, stream, tm,
+                                           dropping ? drop_key : kNoDrop, dropping ? s.n_kept : nullptr);
             if (e != hipSuccess) return e;
             // After a first pass that dropped keys the remaining passes
             // sort the pairs it kept (count on the device).
             if (dropping) n_dev = s.n_kept;
             shift += pass_bits[p];
-            vin = vout;
-            std::swap(kin, kout);
         }
     }
-    // kin now holds the sorted keys (the caller's buffer or keys_b).
-    const int oblocks = (int)std::min<int64_t>(((int64_t)K + kThreads) / kThreads, 4096);
-    launch(tm, KC_OFFSETS, k_offsets, dim3(oblocks), dim3(kThreads), stream, (const uint32_t*)kin, n, n_dev, K,
+    // keys_b now holds the sorted keys.
+    const int oblocks = (int)std::min<int64_t>(((int64_t)K + 512) / 512, 4096);
+    launch(tm, KC_OFFSETS, k_offsets, dim3(oblocks), dim3(512), stream, (const uint32_t*)s.keys_b, n, n_dev, K,
            offsets_out, done);
     return hipGetLastError();
 }
