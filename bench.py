@@ -67,9 +67,9 @@ def algorithmic_bytes(N, W, lay, inf, counts, full, pairs):
     keys, pairs[d] of them kept; or the queried buckets only, pairs[d]
     selected pairs) as the engine logged it.  Full-table days account to
     gen/sort/offsets, subset days to select/subset_sort.  Per radix pass:
-    upsweep reads the keys (4 B each); the first downsweep reads n keys and
-    writes key and id of the kept pairs (4n + 8k), later ones read and write
-    key and id (16k)."""
+    the first upsweep reads the keys (4 B each), later ones the (key, id)
+    pairs (8 B); the first downsweep reads n keys and writes key and id of
+    the kept pairs (4n + 8k), later ones read and write key and id (16k)."""
     per = {}
 
     def add(k, b):
@@ -105,14 +105,14 @@ def algorithmic_bytes(N, W, lay, inf, counts, full, pairs):
             k = int(pairs[d]) if d < len(pairs) and int(pairs[d]) > 0 else n
             add("gen_weekend" if weekend else "gen_weekday", 4 * n + (0 if weekend else N))
             for p in range(passes):
-                add("sort_upsweep", 4 * (k if p else n))
+                add("sort_upsweep", 8 * k if p else 4 * n)    # later passes read (key, id) pairs
                 add("sort_downsweep", (4 * n + 8 * k) if p == 0 else 16 * k)
             add("offsets", 4 * k + 4 * (buckets + 1))
         elif d < len(full):
             m = int(pairs[d])
             add("select", (0 if weekend else N) + 8 * m + 16 * m)   # ages; staged pairs; compaction
             for p in range(passes):
-                add("subset_sort", 4 * m + 16 * m)
+                add("subset_sort", (4 if p == 0 else 8) * m + 16 * m)
             add("subset_sort", 4 * m + 4 * (buckets + 1))
         add("apply", 3 * X + 16 * X)
     return per, loop
