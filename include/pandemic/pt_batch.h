@@ -69,6 +69,13 @@ int pt_batch_set_interventions(pt_batch* b, const pt_interventions* iv);
 /* As pt_sim_set_table_mode; applies to every replicate. */
 int pt_batch_set_table_mode(pt_batch* b, int32_t mode);
 
+/* Day step as HIP graphs (default 1): pt_batch_run captures the weekday and
+ * the weekend step once (again after setup or a table-mode change) and
+ * replays one graph per day; 0 enqueues every kernel of every day instead.
+ * Results are identical either way.  No counterpart in the reference,
+ * which launches each day's kernels from the host (PandemicSim.cu:877-967). */
+int pt_batch_use_graphs(pt_batch* b, int32_t enable);
+
 /* Device-side save / restore of every replicate's post-seeding state. */
 int pt_batch_snapshot(pt_batch* b);
 int pt_batch_restore(pt_batch* b);

@@ -33,6 +33,7 @@ EXPORTS = [
     "pt_mlp_train", "pt_mlp_gradients", "pt_mlp_predict", "pt_mlp_last_train_ms", "pt_mlp_destroy",
     "pt_sim_day_kernel_times", "pt_sim_setup_times", "pt_sim_daily_rows", "pt_sim_checkpoint_bytes",
     "pt_sim_checkpoint_save", "pt_sim_checkpoint_load", "pt_device_memory", "pt_device_name",
+    "pt_batch_use_graphs",
 ]
 
 # enum pt_setup_phase (include/pandemic/pt_abi.h)
@@ -158,6 +159,7 @@ def lib():
         L.pt_sim_checkpoint_load.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
         L.pt_device_memory.argtypes = [C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.pt_device_name.argtypes = [C.c_int32, C.c_char_p, C.c_int32]
+        L.pt_batch_use_graphs.argtypes = [C.c_void_p, C.c_int32]
         _lib = L
     return _lib
 
@@ -435,6 +437,10 @@ class PandemicBatch:
 
     def set_table_mode(self, mode):
         _check(lib().pt_batch_set_table_mode(self.h, mode))
+
+    def use_graphs(self, on=True):
+        """Day step replayed from HIP graphs (default) or enqueued kernel by kernel."""
+        _check(lib().pt_batch_use_graphs(self.h, 1 if on else 0))
 
     def set_interventions(self, ivs):
         """One Interventions per replicate; call before setup()."""

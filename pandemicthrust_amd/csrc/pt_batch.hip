@@ -501,6 +501,12 @@ struct pt_batch {
     TableScratch tables;
     int grid_contacts = 2048;
     hipEvent_t run_begin = nullptr, run_end = nullptr;
+    // The day step captured once per layout as HIP graphs (weekday,
+    // weekend) and replayed for every day; the kernels take the day and
+    // every count from the device, so one capture serves all days.
+    bool use_graphs = true;
+    hipGraphExec_t day_graph[2] = {nullptr, nullptr};
+    int graph_table_mode = -1;
     double last_run_ms = 0;
 };
 
@@ -649,6 +655,39 @@ int enqueue_batch_day(pt_batch* b, int d) {
     return 0;
 }
 
+void drop_day_graphs(pt_batch* b) {
+    for (auto& g : b->day_graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    b->graph_table_mode = -1;
+}
+
+// Captures the weekday (day 0) and weekend (day 5) steps.
+int capture_day_graphs(pt_batch* b) {
+    drop_day_graphs(b);
+    for (int w = 0; w < 2; ++w) {
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_batch_day(b, w ? 5 : 0);
+        const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+        if (rc != 0 || e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            drop_day_graphs(b);
+            return rc != 0 ? rc : fail(PT_EDEVICE, "day capture: %s", hipGetErrorString(e));
+        }
+        const hipError_t ei = hipGraphInstantiate(&b->day_graph[w], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            drop_day_graphs(b);
+            return fail(PT_EDEVICE, "day graph: %s", hipGetErrorString(ei));
+        }
+    }
+    b->graph_table_mode = b->table_mode;
+    return 0;
+}
+
 }  // namespace
 
 // ============================ C ABI ============================
@@ -698,6 +737,7 @@ int pt_batch_create(int32_t n, const pt_params* params, const int32_t* seed_keys
 int pt_batch_setup(pt_batch* b, int64_t* n_people_out) {
     if (!b) return fail(PT_EINVAL, "null handle");
     HIP_TRY(hipSetDevice(b->device));
+    drop_day_graphs(b);  // captured with the buffers freed below
     free_batch_population(b);
     const int R = b->R;
     hipStream_t st = b->stream;
@@ -870,9 +910,15 @@ int pt_batch_run(pt_batch* b, int32_t* counts_out, int32_t* infected_out, int32_
     HIP_TRY(hipMemcpyAsync(&g, b->d_g, sizeof g, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     if (g.day != 0) return fail(PT_ESTATE, "run after run: restore the snapshot first");
+    if (b->use_graphs && (!b->day_graph[0] || b->graph_table_mode != b->table_mode))
+        if (capture_day_graphs(b)) return PT_EDEVICE;
     HIP_TRY(hipEventRecord(b->run_begin, b->stream));
-    for (int d = 0; d < b->D; ++d)
-        if (enqueue_batch_day(b, d)) return PT_EDEVICE;
+    for (int d = 0; d < b->D; ++d) {
+        if (b->use_graphs)
+            HIP_TRY(hipGraphLaunch(b->day_graph[(d % 7) >= 5 ? 1 : 0], b->stream));
+        else if (enqueue_batch_day(b, d))
+            return PT_EDEVICE;
+    }
     HIP_TRY(hipEventRecord(b->run_end, b->stream));
     HIP_TRY(hipEventSynchronize(b->run_end));
     float ms = 0.f;
@@ -970,6 +1016,12 @@ int pt_batch_set_interventions(pt_batch* b, const pt_interventions* iv) {
     return PT_OK;
 }
 
+int pt_batch_use_graphs(pt_batch* b, int32_t enable) {
+    if (!b) return fail(PT_EINVAL, "null handle");
+    b->use_graphs = enable != 0;
+    return PT_OK;
+}
+
 int pt_batch_set_table_mode(pt_batch* b, int32_t mode) {
     if (!b) return fail(PT_EINVAL, "null handle");
     if (mode < TABLES_AUTO || mode > TABLES_SUBSET) return fail(PT_EINVAL, "table mode %d unknown", mode);
@@ -1018,6 +1070,7 @@ void pt_batch_destroy(pt_batch* b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_day_graphs(b);
     free_batch_population(b);
     dev_free(b->d_models);
     dev_free(b->d_reps);

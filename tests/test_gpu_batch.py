@@ -116,6 +116,28 @@ def test_batch_rerun_after_restore_is_identical():
     b.close()
 
 
+def test_graph_replays_match_eager_launches():
+    # The day step replayed from captured graphs (the default) against the
+    # same days enqueued kernel by kernel, over several replays with eager
+    # work and synchronisations in between, in every table mode.
+    for mode in (0, 1, 2):
+        b, n = _batch(ROWS[:5], mode)
+        b.snapshot()
+        b.use_graphs(False)
+        eager = b.run()
+        eager_state = [b.array(r, 0).copy() for r in range(5)]
+        b.use_graphs(True)
+        for rep in range(3):
+            b.restore()
+            got = b.run()
+            for r in range(5):
+                assert np.array_equal(got[r][0], eager[r][0]) and np.array_equal(got[r][1], eager[r][1]), (mode, rep, r)
+                assert np.array_equal(b.array(r, 0), eager_state[r]), (mode, rep, r)
+            b.generation_offsets(rep)          # eager work between replays
+            b.synchronize()
+        b.close()
+
+
 def test_batch_of_one_equals_single_simulation():
     b, n = _batch(ROWS[6:7])
     res = b.run()
