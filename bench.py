@@ -150,18 +150,29 @@ def cpu_baseline():
                       "(%d agents, %.1f s loop)" % (r["setup_s"], r["days"], r["n_people"], r["loop_s"])}
 
 
-def replicate_bench(n_reps, plain=True):
+def replicate_bench(n_reps, plain=True, tune=True, solo_launch=None):
     """256-scenario ensemble (pandemicthrust_amd/data/config2_scenarios.csv),
     each replicate run to its end (180 days or extinction), setup included:
     plain = one single-simulation handle after another; batched = one batch.
-    Every replicate's daily counts must agree between the two."""
-    from pandemicthrust_amd import PandemicBatch, PandemicSim, Params
+    Every replicate's daily counts must agree between the two.  The batch's
+    launch shapes come from the autotuner run on the first 64 rows (untimed);
+    plain runs use solo_launch (the daily loop's tuned shapes)."""
+    from pandemicthrust_amd import PandemicBatch, PandemicSim, Params, autotune
     from pandemicthrust_amd.scenarios import load_config2
     rows = load_config2()[:n_reps]
     params = [Params.from_row(f) for _, f in rows]
     seeds = [s for s, _ in rows]
+    batch_launch, tune_log = None, None
+    if tune:
+        tb = PandemicBatch(params[:64], seeds[:64])
+        tb.setup()
+        batch_launch, tune_log = autotune.tune_batch(tb)
+        tb.close()
+        print("replicates: tuned launch %s" % batch_launch, file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     b = PandemicBatch(params, seeds)
+    for k, v in (batch_launch or {}).items():
+        b.set_launch(k, v)
     n_people = b.setup()
     res = b.run()
     batched_s = time.perf_counter() - t0
@@ -176,13 +187,16 @@ def replicate_bench(n_reps, plain=True):
            "agents": info["agents"], "person_days": person_days,
            "batched_per_s": round(len(rows) / batched_s, 2), "batched_s": round(batched_s, 3),
            "batched_loop_ms": round(batched_loop_ms, 1),
-           "batched_person_days_per_s": round(person_days / (batched_loop_ms * 1e-3), 1)}
+           "batched_person_days_per_s": round(person_days / (batched_loop_ms * 1e-3), 1),
+           "launch": {"batch": batch_launch, "plain": solo_launch, "tuning_ms": tune_log}}
     if not plain:
         return out
     identical = True
     t0 = time.perf_counter()
     for r, (p, s) in enumerate(zip(params, seeds)):
         sim = PandemicSim(p, s)
+        for k, v in (solo_launch or {}).items():
+            sim.set_launch(k, v)
         sim.setup_sim()
         counts, inf = sim.run()
         sim.close()
@@ -300,6 +314,7 @@ def main():
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched replicate leg (for profiling the batch engine)")
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the daily loop of one simulation runs on a single MI355X "
@@ -308,7 +323,8 @@ def main():
     if int(os.environ.get("RANK", "0")) != 0:
         return 0
     if args.batched_only:
-        print(json.dumps({"replicates": replicate_bench(args.replicates, plain=False)}), flush=True)
+        print(json.dumps({"replicates": replicate_bench(args.replicates, plain=False, tune=not args.no_autotune)}),
+              flush=True)
         return 0
 
     from pandemicthrust_amd import Params, PandemicSim
@@ -320,6 +336,11 @@ def main():
     lay = sim.layout()
     W = int(sim.scalars()[2])
     print("setup: %d agents in %.2f s" % (N, setup_s), file=sys.stderr, flush=True)
+    launch, tune_log = sim.launch(), None
+    if not args.no_autotune:
+        from pandemicthrust_amd import autotune
+        launch, tune_log = autotune.tune_sim(sim)
+        print("tuned launch %s" % launch, file=sys.stderr, flush=True)
 
     # The first warmup step times every kernel class (the per-kernel table
     # and the choice of the dominant class); the timed steps carry events
@@ -400,10 +421,11 @@ def main():
                           "achieved": round(loop_bytes * args.steps / loop_s / 1e9, 1),
                           "frac": round(loop_bytes * args.steps / loop_s / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels": table,
+        "launch": {"settings": launch, "tuning_ms": tune_log},
     }
     sim.close()
     if args.replicates > 0:
-        line["replicates"] = replicate_bench(args.replicates)
+        line["replicates"] = replicate_bench(args.replicates, tune=not args.no_autotune, solo_launch=launch)
     if args.ensemble > 0:
         line["ensemble"] = ensemble_bench(args.ensemble, 8, 1000000, 120)
     if args.surrogate > 0:

@@ -214,6 +214,22 @@ int pt_sim_checkpoint_load(pt_sim* sim, const void* buf, int64_t bytes);
 int pt_device_memory(int32_t device, uint64_t* free_bytes, uint64_t* total_bytes);
 int pt_device_name(int32_t device, char* buf, int32_t len);
 
+/* Launch shapes.  They change only speed: results are identical for every
+ * accepted value (the kernels stride over their work, and the radix sort
+ * is stable at any tile size).  pandemicthrust_amd/autotune.py picks them
+ * by measurement. */
+enum pt_launch_knob {
+    PT_LAUNCH_CONTACT_BLOCKS_PER_CU = 0, /* blocks per compute unit of the mark, contact and apply kernels
+                                            (1..64, default 8) */
+    PT_LAUNCH_GEN_BLOCKS_PER_CU,         /* blocks per compute unit of the errand-key kernels (1..64, default 16) */
+    PT_LAUNCH_SORT_WIDE_BITS,            /* radix passes of at least this many bits (at most 8) run 1024-thread
+                                            blocks on 8192-pair tiles, narrower ones 512 on 4096 (1..9,
+                                            default 8; 9 = never) */
+    PT_LAUNCH_KNOBS
+};
+int pt_sim_set_launch(pt_sim* sim, int32_t knob, int32_t value);
+int pt_sim_get_launch(pt_sim* sim, int32_t knob);
+
 /* Blocks until the handle's stream is idle; loop time of the last
  * pt_sim_run in milliseconds (HIP events around the enqueued days). */
 int pt_sim_synchronize(pt_sim* sim);

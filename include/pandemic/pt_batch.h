@@ -76,6 +76,12 @@ int pt_batch_set_table_mode(pt_batch* b, int32_t mode);
  * which launches each day's kernels from the host (PandemicSim.cu:877-967). */
 int pt_batch_use_graphs(pt_batch* b, int32_t enable);
 
+/* Launch shapes (pt_launch_knob of pt_abi.h; PT_LAUNCH_GEN_BLOCKS_PER_CU is
+ * unused here: the batch regenerates keys in its selection kernels).  A
+ * change takes effect at the next pt_batch_run (graphs are recaptured). */
+int pt_batch_set_launch(pt_batch* b, int32_t knob, int32_t value);
+int pt_batch_get_launch(pt_batch* b, int32_t knob);
+
 /* Device-side save / restore of every replicate's post-seeding state. */
 int pt_batch_snapshot(pt_batch* b);
 int pt_batch_restore(pt_batch* b);

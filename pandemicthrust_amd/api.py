@@ -33,8 +33,11 @@ EXPORTS = [
     "pt_mlp_train", "pt_mlp_gradients", "pt_mlp_predict", "pt_mlp_last_train_ms", "pt_mlp_destroy",
     "pt_sim_day_kernel_times", "pt_sim_setup_times", "pt_sim_daily_rows", "pt_sim_checkpoint_bytes",
     "pt_sim_checkpoint_save", "pt_sim_checkpoint_load", "pt_device_memory", "pt_device_name",
-    "pt_batch_use_graphs",
+    "pt_batch_use_graphs", "pt_sim_set_launch", "pt_sim_get_launch", "pt_batch_set_launch", "pt_batch_get_launch",
 ]
+
+# enum pt_launch_knob (include/pandemic/pt_abi.h)
+LAUNCH_KNOBS = ["contact_blocks_per_cu", "gen_blocks_per_cu", "sort_wide_bits"]
 
 # enum pt_setup_phase (include/pandemic/pt_abi.h)
 SETUP_PHASES = ["scale", "push", "population_size", "size_arrays", "households", "workplaces", "status",
@@ -160,6 +163,10 @@ def lib():
         L.pt_device_memory.argtypes = [C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.pt_device_name.argtypes = [C.c_int32, C.c_char_p, C.c_int32]
         L.pt_batch_use_graphs.argtypes = [C.c_void_p, C.c_int32]
+        for f in ("pt_sim_set_launch", "pt_batch_set_launch"):
+            getattr(L, f).argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        for f in ("pt_sim_get_launch", "pt_batch_get_launch"):
+            getattr(L, f).argtypes = [C.c_void_p, C.c_int32]
         _lib = L
     return _lib
 
@@ -268,6 +275,13 @@ class PandemicSim:
     def load_checkpoint(self, data):
         buf = np.frombuffer(data, np.uint8)
         _check(lib().pt_sim_checkpoint_load(self.h, buf.ctypes.data, len(buf)))
+
+    def set_launch(self, name, value):
+        """Launch shape (LAUNCH_KNOBS); speed only, results unchanged."""
+        _check(lib().pt_sim_set_launch(self.h, LAUNCH_KNOBS.index(name), int(value)))
+
+    def launch(self):
+        return {k: _check(lib().pt_sim_get_launch(self.h, i)) for i, k in enumerate(LAUNCH_KNOBS)}
 
     def setup_times(self):
         ms = np.zeros(len(SETUP_PHASES), np.float64)
@@ -437,6 +451,13 @@ class PandemicBatch:
 
     def set_table_mode(self, mode):
         _check(lib().pt_batch_set_table_mode(self.h, mode))
+
+    def set_launch(self, name, value):
+        """Launch shape (LAUNCH_KNOBS); speed only, results unchanged."""
+        _check(lib().pt_batch_set_launch(self.h, LAUNCH_KNOBS.index(name), int(value)))
+
+    def launch(self):
+        return {k: _check(lib().pt_batch_get_launch(self.h, i)) for i, k in enumerate(LAUNCH_KNOBS)}
 
     def use_graphs(self, on=True):
         """Day step replayed from HIP graphs (default) or enqueued kernel by kernel."""

@@ -500,6 +500,8 @@ struct pt_batch {
     unsigned long long* d_scratch64 = nullptr;  // max(R, 512)
     TableScratch tables;
     int grid_contacts = 2048;
+    int cus = 256;                                   // compute units of the device
+    int launch[PT_LAUNCH_KNOBS] = {8, 16, 8};        // pt_launch_knob values
     hipEvent_t run_begin = nullptr, run_end = nullptr;
     // The day step captured once per layout as HIP graphs (weekday,
     // weekend) and replayed for every day; the kernels take the day and
@@ -720,7 +722,8 @@ int pt_batch_create(int32_t n, const pt_params* params, const int32_t* seed_keys
     }
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
-        b->grid_contacts = cus * 8;
+        b->cus = cus;
+    b->grid_contacts = b->cus * b->launch[PT_LAUNCH_CONTACT_BLOCKS_PER_CU];
     if (dev_alloc(&b->d_models, n) || dev_alloc(&b->d_reps, n) || dev_alloc(&b->d_states, n) || dev_alloc(&b->d_g, 1) ||
         dev_alloc(&b->rep_I, n) || dev_alloc(&b->inf_start, (int64_t)n + 1) ||
         dev_alloc(&b->d_scratch64, std::max(n, 512))) {
@@ -827,6 +830,7 @@ int pt_batch_setup(pt_batch* b, int64_t* n_people_out) {
         free_batch_population(b);
         return fail(PT_ENOMEM, "table scratch allocation failed");
     }
+    b->tables.wide_bits = b->launch[PT_LAUNCH_SORT_WIDE_BITS];
     HIP_TRY(hipMemcpyAsync(b->tile_rep, trep.data(), sizeof(int32_t) * tiles, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(b->d_models, b->models.data(), sizeof(Model) * R, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(b->d_reps, b->reps.data(), sizeof(RepInfo) * R, hipMemcpyHostToDevice, st));
@@ -1014,6 +1018,22 @@ int pt_batch_set_interventions(pt_batch* b, const pt_interventions* iv) {
     }
     b->models = m;
     return PT_OK;
+}
+
+int pt_batch_set_launch(pt_batch* b, int32_t knob, int32_t value) {
+    if (!b) return fail(PT_EINVAL, "null handle");
+    if (int rc = check_launch(knob, value)) return rc;
+    b->launch[knob] = value;
+    b->grid_contacts = b->cus * b->launch[PT_LAUNCH_CONTACT_BLOCKS_PER_CU];
+    b->tables.wide_bits = b->launch[PT_LAUNCH_SORT_WIDE_BITS];
+    drop_day_graphs(b);  // captured with the old launch shapes
+    return PT_OK;
+}
+
+int pt_batch_get_launch(pt_batch* b, int32_t knob) {
+    if (!b) return fail(PT_EINVAL, "null handle");
+    if (knob < 0 || knob >= PT_LAUNCH_KNOBS) return fail(PT_EINVAL, "launch setting %d unknown", knob);
+    return b->launch[knob];
 }
 
 int pt_batch_use_graphs(pt_batch* b, int32_t enable) {

@@ -25,6 +25,8 @@ namespace pt {
 
 // Records the error text returned by pt_last_error; returns code.
 int fail(int code, const char* fmt, ...);
+// PT_OK when value is in range for launch setting knob (pt_launch_knob).
+int check_launch(int32_t knob, int32_t value);
 
 #define HIP_TRY(expr)                                                                              \
     do {                                                                                           \

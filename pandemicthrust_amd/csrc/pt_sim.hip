@@ -54,6 +54,14 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
+int check_launch(int32_t knob, int32_t value) {
+    if (knob < 0 || knob >= PT_LAUNCH_KNOBS) return fail(PT_EINVAL, "launch setting %d unknown", knob);
+    const int lo[PT_LAUNCH_KNOBS] = {1, 1, 1}, hi[PT_LAUNCH_KNOBS] = {64, 64, 9};
+    if (value < lo[knob] || value > hi[knob])
+        return fail(PT_EINVAL, "launch setting %d: value %d outside [%d, %d]", knob, value, lo[knob], hi[knob]);
+    return PT_OK;
+}
+
 }  // namespace pt
 
 namespace {
@@ -767,6 +775,8 @@ struct pt_sim {
     double setup_ms[PT_SETUP_PHASES] = {0};
     double last_run_ms = 0;
     int grid_contacts = 1024, grid_gen = 2048;
+    int cus = 128;                                   // compute units of the device
+    int launch[PT_LAUNCH_KNOBS] = {8, 16, 8};        // pt_launch_knob values
 };
 
 namespace {
@@ -1045,10 +1055,10 @@ int pt_sim_create(const pt_params* p, const int32_t seed_key[2], int32_t device,
         return fail(PT_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
     }
     int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0) {
-        s->grid_contacts = cus * 8;
-        s->grid_gen = cus * 16;
-    }
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        s->cus = cus;
+    s->grid_contacts = s->cus * s->launch[PT_LAUNCH_CONTACT_BLOCKS_PER_CU];
+    s->grid_gen = s->cus * s->launch[PT_LAUNCH_GEN_BLOCKS_PER_CU];
     if (dev_alloc(&s->d_model, 1) || dev_alloc(&s->d_state, 1) || dev_alloc(&s->d_counts, 256 * 16) ||
         dev_alloc(&s->d_inf, 256) || dev_alloc(&s->d_scratch64, 512) || dev_alloc(&s->d_tab_full, 256) ||
         dev_alloc(&s->d_tab_pairs, 256)) {
@@ -1119,6 +1129,7 @@ int pt_sim_setup(pt_sim* s, int64_t* n_people_out) {
         free_population(s);
         return fail(PT_ENOMEM, "table scratch allocation failed");
     }
+    s->tables.wide_bits = s->launch[PT_LAUNCH_SORT_WIDE_BITS];
     const double sizing_ms = ms_between(t_sized0, std::chrono::steady_clock::now());
     // K3-K7 (the keys buffer is the scratch; the rand offset after the
     // households is the workplace counter, PandemicSim.cu:2570-2571, :3817).
@@ -1578,6 +1589,22 @@ int pt_sim_set_interventions(pt_sim* s, const pt_interventions* iv) {
     if (rc != PT_OK) return rc;
     s->model = m;
     return PT_OK;
+}
+
+int pt_sim_set_launch(pt_sim* s, int32_t knob, int32_t value) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    if (int rc = check_launch(knob, value)) return rc;
+    s->launch[knob] = value;
+    s->grid_contacts = s->cus * s->launch[PT_LAUNCH_CONTACT_BLOCKS_PER_CU];
+    s->grid_gen = s->cus * s->launch[PT_LAUNCH_GEN_BLOCKS_PER_CU];
+    s->tables.wide_bits = s->launch[PT_LAUNCH_SORT_WIDE_BITS];
+    return PT_OK;
+}
+
+int pt_sim_get_launch(pt_sim* s, int32_t knob) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    if (knob < 0 || knob >= PT_LAUNCH_KNOBS) return fail(PT_EINVAL, "launch setting %d unknown", knob);
+    return s->launch[knob];
 }
 
 int pt_sim_set_table_mode(pt_sim* s, int32_t mode) {

@@ -84,6 +84,10 @@ struct TableScratch {
     // Keys come in long runs of equal high bits (tables of many simulations
     // side by side): count digits with wave-aggregated atomics.
     bool grouped_keys = false;
+    // Radix passes of at least this many bits (at most 8) run 1024-thread
+    // blocks on 8192-pair tiles, narrower ones 512-thread blocks on
+    // 4096-pair tiles; 9 keeps every pass at 512 threads.
+    int wide_bits = 8;
 };
 
 int table_scratch_alloc(TableScratch& s, int64_t n_cap);

@@ -10,8 +10,9 @@
 // or implied by the position) and writes interleaved (key, id) pairs, one
 // 8-byte store per pair; middle passes read and write pairs; the last pass
 // writes keys and ids to separate arrays (ids are the table, keys feed the
-// bucket offsets).  Full 8-bit passes run 1024-thread blocks (8192-pair
-// tiles, so digit runs average 32 pairs), narrower digits 512-thread blocks.
+// bucket offsets).  Passes of at least TableScratch::wide_bits bits (8 by
+// default) run 1024-thread blocks on 8192-pair tiles (digit runs of ~32
+// pairs), the others 512-thread blocks on 4096-pair tiles.
 #include "pt_sort.h"
 
 #include <algorithm>
@@ -34,11 +35,6 @@ constexpr int kHistBlocks = 1024;  // most blocks of any pass
 template <int T>
 constexpr int max_blocks() {
     return T >= 1024 ? 256 : 1024;
-}
-// Block width of a pass with BITS-bit digits.
-template <int BITS>
-constexpr int pass_threads() {
-    return BITS == 8 ? 1024 : 512;
 }
 
 int g_max_bits = 8;  // widest digit per pass (see set_radix_max_bits)
@@ -443,11 +439,10 @@ struct PassIO {
     uint2* pout;
 };
 
-template <int BITS>
+template <int BITS, int T>
 hipError_t radix_pass(TableScratch& s, const PassIO& io, int id_mode, int32_t N, int64_t n, const uint32_t* n_dev,
                       int shift, const int32_t* done, hipStream_t st, LaunchTimer* tm, uint32_t drop,
                       uint32_t* n_out) {
-    constexpr int T = pass_threads<BITS>();
     const int64_t tiles = (n + (int64_t)T * kItems - 1) / ((int64_t)T * kItems);
     const int nblk = (int)std::min<int64_t>(std::max<int64_t>(tiles, 1), max_blocks<T>());
     const bool pin = io.pin != nullptr;
@@ -482,16 +477,23 @@ hipError_t radix_pass(TableScratch& s, const PassIO& io, int id_mode, int32_t N,
     return hipGetLastError();
 }
 
+// 1024-thread blocks exist for digits of up to 8 bits (LDS).
 hipError_t radix_pass_bits(int bits, TableScratch& s, const PassIO& io, int id_mode, int32_t N, int64_t n,
                            const uint32_t* n_dev, int shift, const int32_t* done, hipStream_t st, LaunchTimer* tm,
                            uint32_t drop, uint32_t* n_out) {
-#define PT_PASS(B) \
-    case B: return radix_pass<B>(s, io, id_mode, N, n, n_dev, shift, done, st, tm, drop, n_out);
+    const bool wide = bits <= 8 && bits >= s.wide_bits;
+#define PT_PASS2(B)                                                                                          \
+    case B:                                                                                                  \
+        return wide ? radix_pass<B, 1024>(s, io, id_mode, N, n, n_dev, shift, done, st, tm, drop, n_out)     \
+                    : radix_pass<B, 512>(s, io, id_mode, N, n, n_dev, shift, done, st, tm, drop, n_out);
+#define PT_PASS1(B) \
+    case B: return radix_pass<B, 512>(s, io, id_mode, N, n, n_dev, shift, done, st, tm, drop, n_out);
     switch (bits) {
-        PT_PASS(1) PT_PASS(2) PT_PASS(3) PT_PASS(4) PT_PASS(5) PT_PASS(6)
-        PT_PASS(7) PT_PASS(8) PT_PASS(9) PT_PASS(10) PT_PASS(11)
+        PT_PASS2(1) PT_PASS2(2) PT_PASS2(3) PT_PASS2(4) PT_PASS2(5) PT_PASS2(6)
+        PT_PASS2(7) PT_PASS2(8) PT_PASS1(9) PT_PASS1(10) PT_PASS1(11)
     }
-#undef PT_PASS
+#undef PT_PASS1
+#undef PT_PASS2
     return hipErrorInvalidValue;
 }
 
@@ -508,6 +510,7 @@ void set_radix_max_bits(int bits) { g_max_bits = std::min(std::max(bits, 4), kMa
 int radix_passes(int32_t K) { return (key_bits(K) + g_max_bits - 1) / g_max_bits; }
 
 int table_scratch_alloc(TableScratch& s, int64_t n_cap) {
+    const int wide_bits = s.wide_bits;  // a launch setting, kept across reallocation
     table_scratch_free(s);
     const int64_t n = std::max<int64_t>(n_cap, 1);
     const int64_t hist_n = ((int64_t)1 << kMaxBitsSupported) * kHistBlocks + 1;
@@ -520,6 +523,7 @@ int table_scratch_alloc(TableScratch& s, int64_t n_cap) {
     const int64_t parts = (std::max<int64_t>(n, hist_n) + kScanSeg - 1) / kScanSeg + 2;
     if (hipMalloc(&s.scan_part, (size_t)parts * sizeof(uint32_t)) != hipSuccess) return -1;
     s.n_cap = n_cap;
+    s.wide_bits = wide_bits;
     return 0;
 }
 
