@@ -121,7 +121,12 @@ enum pt_array {
     PT_INFECTED,        /* int32[I] infected_indexes of the last day run */
     PT_ERRAND_OFFSETS,  /* int32[hours*W+1] errand_locationOffsets of the last day, reference layout */
     PT_ERRAND_PEOPLE,   /* int32[E*N] sorted errand table of the last day */
-    PT_HH_TYPES         /* uint8[H+1] household types */
+    PT_HH_TYPES,        /* uint8[H+1] household types */
+    PT_WORKPLACE,       /* int32[N] workplace or school of each agent (validation; people_workplaces) */
+    PT_ERRANDS,         /* int32[E*N] the last day's errand destinations as hour * W + location
+                           (validation; people_errands: weekday hour-major, weekend 3 per person) */
+    PT_LOCATION_TYPE_OFFSETS /* int32[15] first location of each of the 14 location types, then W
+                                (WORKPLACE_TYPE_OFFSET_HOST, the scan of PandemicSim.cu:1155-1158) */
 };
 int64_t pt_sim_array_bytes(pt_sim* sim, int32_t which);
 int pt_sim_copy_array(pt_sim* sim, int32_t which, void* dst, int64_t bytes);
@@ -213,6 +218,22 @@ int pt_sim_checkpoint_load(pt_sim* sim, const void* buf, int64_t bytes);
  * (cudaMemGetInfo, resource_logging.cu:37, :64). */
 int pt_device_memory(int32_t device, uint64_t* free_bytes, uint64_t* total_bytes);
 int pt_device_name(int32_t device, char* buf, int32_t len);
+
+/* Validation trace (SIM_VALIDATION, PandemicSim.h:50-58, debug.cu:138-442,
+ * :847-950): with tracing on, each contact day records per infector
+ * position and slot (C = 8 weekday / 5 weekend) the victim, the contact type
+ * (CONTACT_TYPE_*, simParameters.h:59-64), the contact's location
+ * (household, workplace or school, after-school location, errands as
+ * hour * W + location, -1 for none) and the action (ACTION_INFECT_*,
+ * simParameters.h:23-26: 2 pandemic, 1 seasonal, 3 both, for the attempts
+ * that infected the victim in the canonical order).  Costs 10 bytes per slot
+ * of device memory (8N slots); results are unchanged.  pt_sim_trace copies
+ * the last day's pt_sim_trace_size entries; infector = the infected agent of
+ * the slot's position.  Returns the number of entries. */
+int pt_sim_enable_trace(pt_sim* sim, int32_t on);
+int64_t pt_sim_trace_size(pt_sim* sim);
+int pt_sim_trace(pt_sim* sim, int32_t* infector, int32_t* victim, uint8_t* type, int32_t* loc, uint8_t* action,
+                 int64_t n);
 
 /* Launch shapes.  They change only speed: results are identical for every
  * accepted value (the kernels stride over their work, and the radix sort

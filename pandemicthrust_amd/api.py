@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libpt_hip.so")
 
 PT_STATUS_P, PT_STATUS_S, PT_DAY_P, PT_DAY_S, PT_GEN_P, PT_GEN_S = range(6)
 PT_AGE, PT_HOUSEHOLD, PT_HH_OFFSETS, PT_WP_OFFSETS, PT_WP_PEOPLE = range(6, 11)
-PT_INFECTED, PT_ERRAND_OFFSETS, PT_ERRAND_PEOPLE, PT_HH_TYPES = range(11, 15)
+PT_INFECTED, PT_ERRAND_OFFSETS, PT_ERRAND_PEOPLE, PT_HH_TYPES, PT_WORKPLACE, PT_ERRANDS = range(11, 17)
+PT_LOCATION_TYPE_OFFSETS = 17
 _U8_ARRAYS = {PT_STATUS_P, PT_STATUS_S, PT_DAY_P, PT_DAY_S, PT_GEN_P, PT_GEN_S, PT_AGE, PT_HH_TYPES}
 
 EXPORTS = [
@@ -34,6 +35,7 @@ EXPORTS = [
     "pt_sim_day_kernel_times", "pt_sim_setup_times", "pt_sim_daily_rows", "pt_sim_checkpoint_bytes",
     "pt_sim_checkpoint_save", "pt_sim_checkpoint_load", "pt_device_memory", "pt_device_name",
     "pt_batch_use_graphs", "pt_sim_set_launch", "pt_sim_get_launch", "pt_batch_set_launch", "pt_batch_get_launch",
+    "pt_sim_enable_trace", "pt_sim_trace_size", "pt_sim_trace",
 ]
 
 # enum pt_launch_knob (include/pandemic/pt_abi.h)
@@ -163,6 +165,10 @@ def lib():
         L.pt_device_memory.argtypes = [C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.pt_device_name.argtypes = [C.c_int32, C.c_char_p, C.c_int32]
         L.pt_batch_use_graphs.argtypes = [C.c_void_p, C.c_int32]
+        L.pt_sim_enable_trace.argtypes = [C.c_void_p, C.c_int32]
+        L.pt_sim_trace_size.restype = C.c_int64
+        L.pt_sim_trace_size.argtypes = [C.c_void_p]
+        L.pt_sim_trace.argtypes = [C.c_void_p] + [C.c_void_p] * 5 + [C.c_int64]
         for f in ("pt_sim_set_launch", "pt_batch_set_launch"):
             getattr(L, f).argtypes = [C.c_void_p, C.c_int32, C.c_int32]
         for f in ("pt_sim_get_launch", "pt_batch_get_launch"):
@@ -275,6 +281,22 @@ class PandemicSim:
     def load_checkpoint(self, data):
         buf = np.frombuffer(data, np.uint8)
         _check(lib().pt_sim_checkpoint_load(self.h, buf.ctypes.data, len(buf)))
+
+    def enable_trace(self, on=True):
+        """Per-contact validation trace of every contact day (pt_sim_trace)."""
+        _check(lib().pt_sim_enable_trace(self.h, 1 if on else 0))
+
+    def trace(self):
+        """The last contact day's trace: infector, victim, type, loc, action per slot."""
+        n = int(lib().pt_sim_trace_size(self.h))
+        if n < 0:
+            raise PTError(lib().pt_last_error().decode())
+        out = {"infector": np.zeros(n, np.int32), "victim": np.zeros(n, np.int32), "type": np.zeros(n, np.uint8),
+               "loc": np.zeros(n, np.int32), "action": np.zeros(n, np.uint8)}
+        if n:
+            _check(lib().pt_sim_trace(self.h, out["infector"].ctypes.data, out["victim"].ctypes.data,
+                                      out["type"].ctypes.data, out["loc"].ctypes.data, out["action"].ctypes.data, n))
+        return out
 
     def set_launch(self, name, value):
         """Launch shape (LAUNCH_KNOBS); speed only, results unchanged."""

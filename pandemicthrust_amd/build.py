@@ -44,7 +44,7 @@ def build_hip():
 def build_host():
     os.makedirs(os.path.join(HERE, "bin"), exist_ok=True)
     out = os.path.join(HERE, "bin", "pandemicthrust")
-    src = [os.path.join(HERE, "host", f) for f in ("main.cpp", "pandemic_sim.cpp")]
+    src = [os.path.join(HERE, "host", f) for f in ("main.cpp", "pandemic_sim.cpp", "validation.cpp")]
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", out] + src +
          ["-L" + os.path.join(HERE, "lib"), "-lpt_hip", "-Wl,-rpath,$ORIGIN/../lib"])
     return out

@@ -92,6 +92,8 @@ __device__ __forceinline__ AgentArrays view(const BatchArrays& B, const RepInfo&
     A.claim_s = B.claim_s + ri.abase;
     A.trace_victim = nullptr;
     A.trace_type = nullptr;
+    A.trace_loc = nullptr;
+    A.trace_attempt = nullptr;
     return A;
 }
 

@@ -443,8 +443,18 @@ struct AgentArrays {
     const uint8_t* gn_s;
     uint64_t* claim_p;
     uint64_t* claim_s;
-    int32_t* trace_victim;     // optional [I*C]
-    uint8_t* trace_type;       // optional [I*C]
+    // Optional per-contact trace of the day (validation mode), [I * C]
+    // with C = 8 weekday / 5 weekend slots per infector: victim, type, the
+    // contact's location (household, workplace, location id for
+    // after-school, hour * W + location for errands; also for a contact that
+    // found nobody else there, whose victim is the location's count; -1 for
+    // an unused or dropped slot) and the
+    // attempts that succeeded on a susceptible victim (2 pandemic, 1
+    // seasonal; which of them won the victim is settled after all claims).
+    int32_t* trace_victim;
+    uint8_t* trace_type;
+    int32_t* trace_loc;
+    uint8_t* trace_attempt;
 };
 
 // Strain state of an infector (device_processContacts, PandemicSim.cu:3125-3175):
@@ -510,12 +520,14 @@ __device__ __forceinline__ void process_contacts(const Model& M, const AgentArra
     for (int c = 0; c < C; ++c) {
         const float kv = typ[c] == CT_NONE ? 0.f : 1.f;
         const int32_t v = vic[c];
+        uint8_t attempt = 0;
         if (unit_float(u[2 * c]) < prob_p * kv) {
             if (A.st_p[v] == kSusceptible) {
                 const uint32_t prof = infection_profile(M, u[2 * c] ^ u[(2 * c + 1) % 16]);
                 const uint64_t claim = ((uint64_t)(pos * 8u + (uint32_t)c) << 32) | ((uint32_t)f.gen_p << 8) | prof;
                 const uint64_t old = atomicMin((unsigned long long*)&A.claim_p[v], (unsigned long long)claim);
                 newp[c] = old == ~0ull;
+                attempt |= 2;
             }
         }
         if (unit_float(u[2 * c + 1]) < prob_s * kv) {
@@ -524,8 +536,10 @@ __device__ __forceinline__ void process_contacts(const Model& M, const AgentArra
                 const uint64_t claim = ((uint64_t)(pos * 8u + (uint32_t)c) << 32) | ((uint32_t)f.gen_s << 8) | prof;
                 const uint64_t old = atomicMin((unsigned long long*)&A.claim_s[v], (unsigned long long)claim);
                 news[c] = old == ~0ull;
+                attempt |= 1;
             }
         }
+        if (A.trace_attempt) A.trace_attempt[(int64_t)pos * C + c] = attempt;
     }
 }
 
@@ -600,22 +614,28 @@ __device__ __forceinline__ void weekday_contacts(const Model& M, const AgentArra
     const int wtype = adult ? CT_WORKPLACE : CT_SCHOOL;
     const int etype = adult ? CT_ERRAND : CT_AFTERSCHOOL;
     int kval = 0;
+    int32_t where[8];  // the contact's location (trace only)
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         int t;
         if (c < 3) {
             t = select_person(idx, hoff, hcnt, u[c], CT_HOME, nullptr, vic[c]);
+            where[c] = hh;
         } else if (c < 3 + wpd) {
             t = select_person(idx, woff, wcnt, u[c], wtype, A.wp_people, vic[c]);
+            where[c] = wp;
         } else {
             const int e = c - 3 - wpd;
-            if (e < h0)
+            if (e < h0) {
                 t = select_person(idx, e0off, e0cnt, u[c], etype, A.er_people, vic[c]);
-            else if (e < h0 + h1)
+                where[c] = d1;
+            } else if (e < h0 + h1) {
                 t = select_person(idx, e1off, e1cnt, u[c], etype, A.er_people, vic[c]);
-            else {
+                where[c] = adult ? M.W + d2 : d2;
+            } else {
                 vic[c] = -1;
                 t = CT_NONE;
+                where[c] = -1;
             }
         }
         typ[c] = t;
@@ -627,9 +647,13 @@ __device__ __forceinline__ void weekday_contacts(const Model& M, const AgentArra
         for (int c = 0; c < 8; ++c) {
             A.trace_victim[pos * 8 + c] = vic[c];
             A.trace_type[pos * 8 + c] = (uint8_t)typ[c];
+            A.trace_loc[pos * 8 + c] = vic[c] < 0 ? -1 : where[c];
         }
     }
-    if (can_infect(M, f)) process_contacts<8>(M, A, f, kval, vic, typ, Mc + 2, (uint32_t)pos, newp, news);
+    if (can_infect(M, f))
+        process_contacts<8>(M, A, f, kval, vic, typ, Mc + 2, (uint32_t)pos, newp, news);
+    else if (A.trace_attempt)
+        for (int c = 0; c < 8; ++c) A.trace_attempt[pos * 8 + c] = 0;
 }
 
 // D6e + D7: one infector of kernel_doWeekend (PandemicSim.cu:3311-3379) with
@@ -651,6 +675,7 @@ __device__ __forceinline__ void weekend_contacts(const Model& M, const AgentArra
     draw2x32((uint32_t)(Mc + 1), (uint32_t)M.key0, (uint32_t)M.key1, v32);
     int32_t hour[3], loc[3];
     weekend_errands(M, eoff + (uint64_t)(int64_t)(2 * idx), hour, loc);
+    int32_t where[5] = {hh, hh, hh, -1, -1};  // the contact's location (trace only)
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int slot = M.we_assign[prof][k];
@@ -660,6 +685,7 @@ __device__ __forceinline__ void weekend_contacts(const Model& M, const AgentArra
         const int32_t key = hsel * M.we_E + l;
         const int32_t eoff2 = A.er_off[key], ecnt = A.er_off[key + 1] - eoff2;
         typ[3 + k] = select_person(idx, eoff2, ecnt, v32[k], CT_ERRAND, A.er_people, vic[3 + k]);
+        where[3 + k] = hsel * M.W + lsel;
     }
     int kval = 0;
 #pragma unroll
@@ -670,9 +696,13 @@ __device__ __forceinline__ void weekend_contacts(const Model& M, const AgentArra
         for (int c = 0; c < 5; ++c) {
             A.trace_victim[pos * 5 + c] = vic[c];
             A.trace_type[pos * 5 + c] = (uint8_t)typ[c];
+            A.trace_loc[pos * 5 + c] = vic[c] < 0 ? -1 : where[c];
         }
     }
-    if (can_infect(M, f)) process_contacts<5>(M, A, f, kval, vic, typ, Mc + 2, (uint32_t)pos, newp, news);
+    if (can_infect(M, f))
+        process_contacts<5>(M, A, f, kval, vic, typ, Mc + 2, (uint32_t)pos, newp, news);
+    else if (A.trace_attempt)
+        for (int c = 0; c < 5; ++c) A.trace_attempt[pos * 5 + c] = 0;
 }
 
 // ---------------- queried-bucket marks ----------------

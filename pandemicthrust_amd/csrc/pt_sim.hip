@@ -490,6 +490,60 @@ __global__ void k_end_day(DayState* st, int32_t* __restrict__ tab_full, int32_t*
     if (st->day >= st->max_days) st->done = 1;
 }
 
+// Validation trace: which successful attempts won their victim.  A claim
+// word holds the smallest (position * 8 + slot) of the day's attempts on
+// the victim, the canonical winner (the oracle's sequential order).
+// Pandemic winner -> 2, seasonal winner -> 1 (ACTION_INFECT_* codes,
+// simParameters.h:23-26).  Runs before k_apply clears the claim words.
+__global__ void k_trace_actions(const DayState* __restrict__ st, int C, const int32_t* __restrict__ victim,
+                                const uint8_t* __restrict__ attempt, const uint64_t* __restrict__ claim_p,
+                                const uint64_t* __restrict__ claim_s, uint8_t* __restrict__ action) {
+    if (st->done) return;
+    const int64_t n = (int64_t)st->n_infected * C;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t key = (uint32_t)(e / C) * 8u + (uint32_t)(e % C);
+        const uint8_t a = attempt[e];
+        uint8_t act = 0;
+        if ((a & 2) && (uint32_t)(claim_p[victim[e]] >> 32) == key) act += 2;
+        if ((a & 1) && (uint32_t)(claim_s[victim[e]] >> 32) == key) act += 1;
+        action[e] = act;
+    }
+}
+
+// Per-agent workplace or school (device_recalcWorkplace, PandemicSim.cu:3872-3890).
+__global__ void k_workplace_of(const Model* __restrict__ Mp, uint64_t workplace_off, const uint8_t* __restrict__ age,
+                               int64_t N, int32_t* __restrict__ out) {
+    const Model& M = *Mp;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const U4 u = draw4(workplace_off + (uint64_t)(i / 4), M.key0, M.key1);
+        uint8_t a = age[i];
+        out[i] = assign_workplace(M, word(u, (int)(i % 4)), &a);
+    }
+}
+
+// The day's errand destinations in the reference's encoding hour * W +
+// location (people_errands): weekday hour-major [hour 0: N][hour 1: N]
+// (PandemicSim.cu:2987-3020), weekend 3 per person (:2082-2094).
+__global__ void k_errands_of(const Model* __restrict__ Mp, const DayState* __restrict__ st, int weekend,
+                             const uint8_t* __restrict__ age, int64_t N, int32_t* __restrict__ out) {
+    const Model& M = *Mp;
+    const uint64_t eoff = st->errand_off;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        if (weekend) {
+            int32_t hour[3], loc[3];
+            weekend_errands(M, eoff + (uint64_t)(2 * i), hour, loc);
+            for (int j = 0; j < 3; ++j) out[3 * i + j] = hour[j] * M.W + loc[j];
+        } else {
+            const U4 ue = draw4(eoff + (uint64_t)(i / 2), M.key0, M.key1);
+            const int rs = 2 * (int)(i % 2);
+            int32_t d1, d2;
+            weekday_dests(M, word(ue, rs), word(ue, rs + 1), age[i] == kAgeAdult, d1, d2);
+            out[i] = d1;
+            out[N + i] = M.W + d2;
+        }
+    }
+}
+
 // ============================ host side ============================
 
 // The reference model tables (PandemicSim.cu:329-481; simParameters.h).
@@ -775,6 +829,10 @@ struct pt_sim {
     double setup_ms[PT_SETUP_PHASES] = {0};
     double last_run_ms = 0;
     int grid_contacts = 1024, grid_gen = 2048;
+    // Validation trace of the last contact day (pt_sim_enable_trace).
+    bool trace_on = false;
+    int32_t *tr_victim = nullptr, *tr_loc = nullptr;
+    uint8_t *tr_type = nullptr, *tr_attempt = nullptr, *tr_action = nullptr;
     int cus = 128;                                   // compute units of the device
     int launch[PT_LAUNCH_KNOBS] = {8, 16, 8};        // pt_launch_knob values
 };
@@ -791,7 +849,27 @@ double event_ms(hipEvent_t a, hipEvent_t b) {
 }
 
 // Buffers sized by the population (allocated in pt_sim_setup).
+void free_trace(pt_sim* s) {
+    dev_free(s->tr_victim);
+    dev_free(s->tr_type);
+    dev_free(s->tr_loc);
+    dev_free(s->tr_attempt);
+    dev_free(s->tr_action);
+}
+
+int alloc_trace(pt_sim* s) {
+    free_trace(s);
+    const int64_t n = 8 * s->N;
+    if (dev_alloc(&s->tr_victim, n) || dev_alloc(&s->tr_type, n) || dev_alloc(&s->tr_loc, n) ||
+        dev_alloc(&s->tr_attempt, n) || dev_alloc(&s->tr_action, n)) {
+        free_trace(s);
+        return PT_ENOMEM;
+    }
+    return 0;
+}
+
 void free_population(pt_sim* s) {
+    free_trace(s);
     dev_free(s->st_p);
     dev_free(s->st_s);
     dev_free(s->dy_p);
@@ -828,7 +906,7 @@ void free_population(pt_sim* s) {
     s->setup_done = s->seeded = s->have_snapshot = false;
 }
 
-AgentArrays agent_arrays(pt_sim* s, int32_t* trace_victim, uint8_t* trace_type) {
+AgentArrays agent_arrays(pt_sim* s) {
     AgentArrays A;
     A.age = s->age;
     A.household = s->household;
@@ -845,8 +923,10 @@ AgentArrays agent_arrays(pt_sim* s, int32_t* trace_victim, uint8_t* trace_type) 
     A.gn_s = s->gn_s;
     A.claim_p = s->claim_p;
     A.claim_s = s->claim_s;
-    A.trace_victim = trace_victim;
-    A.trace_type = trace_type;
+    A.trace_victim = s->trace_on ? s->tr_victim : nullptr;
+    A.trace_type = s->trace_on ? s->tr_type : nullptr;
+    A.trace_loc = s->trace_on ? s->tr_loc : nullptr;
+    A.trace_attempt = s->trace_on ? s->tr_attempt : nullptr;
     return A;
 }
 
@@ -873,7 +953,7 @@ int enqueue_day(pt_sim* s, int d) {
     const int64_t n = weekend ? 3 * s->N : 2 * s->N;
     const int32_t K = weekend ? kWeHours * s->model.we_E : s->model.wd_K;
     const IdMode mode = weekend ? ID_DIV3 : ID_HOUR_MAJOR;
-    AgentArrays A = agent_arrays(s, nullptr, nullptr);
+    AgentArrays A = agent_arrays(s);
     const Model* M = s->d_model;
     const int32_t* infected = s->infected;
     // Which buckets will be queried today, and which table path to take.
@@ -924,6 +1004,12 @@ int enqueue_day(pt_sim* s, int d) {
     else
         launch(tm, KC_CONTACTS_WEEKDAY, k_contacts<false>, dim3(s->grid_contacts), dim3(kContactThreads), stc, M, st,
                infected, A, s->list_p, s->list_s, s->workplace_off);
+    if (s->trace_on)
+        k_trace_actions<<<s->grid_contacts, 256, 0, stc>>>((const DayState*)st, weekend ? 5 : 8,
+                                                            (const int32_t*)s->tr_victim,
+                                                            (const uint8_t*)s->tr_attempt,
+                                                            (const uint64_t*)s->claim_p,
+                                                            (const uint64_t*)s->claim_s, s->tr_action);
     launch(tm, KC_APPLY, k_apply, dim3(s->grid_contacts), dim3(256), stc, (const DayState*)st, s->claim_p, s->claim_s,
            (const int32_t*)s->list_p, (const int32_t*)s->list_s, s->st_p, s->st_s, s->dy_p, s->dy_s, s->gn_p, s->gn_s);
     launch(tm, KC_END_DAY, k_end_day, dim3(1), dim3(1), stc, st, s->d_tab_full, s->d_tab_pairs, (int32_t)n,
@@ -1130,6 +1216,10 @@ int pt_sim_setup(pt_sim* s, int64_t* n_people_out) {
         return fail(PT_ENOMEM, "table scratch allocation failed");
     }
     s->tables.wide_bits = s->launch[PT_LAUNCH_SORT_WIDE_BITS];
+    if (s->trace_on && alloc_trace(s)) {
+        free_population(s);
+        return fail(PT_ENOMEM, "trace allocation failed");
+    }
     const double sizing_ms = ms_between(t_sized0, std::chrono::steady_clock::now());
     // K3-K7 (the keys buffer is the scratch; the rand offset after the
     // households is the workplace counter, PandemicSim.cu:2570-2571, :3817).
@@ -1277,6 +1367,11 @@ int64_t pt_sim_array_bytes(pt_sim* s, int32_t which) {
             return 4 * ((int64_t)(s->last_weekend ? kWeHours : kWdHours) * s->W + 1);
         case PT_ERRAND_PEOPLE: return s->last_weekend < 0 ? 0 : 4 * s->last_table_n;
         case PT_HH_TYPES: return (int64_t)s->H + 1;
+        case PT_WORKPLACE: return 4 * N;
+        case PT_ERRANDS:
+            if (s->last_weekend < 0) return 0;
+            return 4 * N * (s->last_weekend ? 3 : 2);
+        case PT_LOCATION_TYPE_OFFSETS: return 4 * (kTypes + 1);
     }
     return -1;
 }
@@ -1306,6 +1401,26 @@ int pt_sim_copy_array(pt_sim* s, int32_t which, void* dst, int64_t bytes) {
         case PT_WP_PEOPLE: src = s->wp_people; break;
         case PT_INFECTED: src = s->infected; break;
         case PT_HH_TYPES: src = s->hh_types; break;
+        case PT_WORKPLACE:
+        case PT_ERRANDS: {
+            // Computed into the key scratch (3N words, free between days).
+            if (which == PT_WORKPLACE)
+                k_workplace_of<<<grid_for(s->N, 256, 8192), 256, 0, s->stream>>>(s->d_model, s->workplace_off, s->age,
+                                                                                s->N, (int32_t*)s->keys);
+            else
+                k_errands_of<<<grid_for(s->N, 256, 8192), 256, 0, s->stream>>>(
+                    s->d_model, (const DayState*)s->d_state, s->last_weekend, s->age, s->N, (int32_t*)s->keys);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(s->stream));
+            src = s->keys;
+            break;
+        }
+        case PT_LOCATION_TYPE_OFFSETS: {
+            int32_t* out = (int32_t*)dst;
+            for (int t = 0; t < kTypes; ++t) out[t] = s->model.wp_offset[t];
+            out[kTypes] = s->W;
+            return PT_OK;
+        }
         case PT_ERRAND_PEOPLE:
             if (s->last_weekend == 1) {
                 src = s->er_people;
@@ -1589,6 +1704,49 @@ int pt_sim_set_interventions(pt_sim* s, const pt_interventions* iv) {
     if (rc != PT_OK) return rc;
     s->model = m;
     return PT_OK;
+}
+
+int pt_sim_enable_trace(pt_sim* s, int32_t on) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (!on) {
+        free_trace(s);
+        s->trace_on = false;
+        return PT_OK;
+    }
+    if (s->setup_done && !s->tr_victim && alloc_trace(s)) return fail(PT_ENOMEM, "trace allocation failed");
+    s->trace_on = true;
+    return PT_OK;
+}
+
+int64_t pt_sim_trace_size(pt_sim* s) {
+    if (!s || !s->setup_done || !s->trace_on || s->host_day == 0 || s->last_weekend < 0) return 0;
+    DayState h;
+    if (read_state(s, &h)) return -1;
+    return (int64_t)h.n_infected * (s->last_weekend ? 5 : 8);
+}
+
+int pt_sim_trace(pt_sim* s, int32_t* infector, int32_t* victim, uint8_t* type, int32_t* loc, uint8_t* action,
+                 int64_t n) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    const int64_t have = pt_sim_trace_size(s);
+    if (have < 0) return PT_EDEVICE;
+    if (n < have) return fail(PT_EINVAL, "trace holds %lld entries, room for %lld", (long long)have, (long long)n);
+    if (have == 0) return 0;
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const int C = s->last_weekend ? 5 : 8;
+    if (infector) {
+        std::vector<int32_t> inf((size_t)(have / C));
+        HIP_TRY(hipMemcpy(inf.data(), s->infected, inf.size() * 4, hipMemcpyDeviceToHost));
+        for (int64_t e = 0; e < have; ++e) infector[e] = inf[(size_t)(e / C)];
+    }
+    if (victim) HIP_TRY(hipMemcpy(victim, s->tr_victim, (size_t)have * 4, hipMemcpyDeviceToHost));
+    if (type) HIP_TRY(hipMemcpy(type, s->tr_type, (size_t)have, hipMemcpyDeviceToHost));
+    if (loc) HIP_TRY(hipMemcpy(loc, s->tr_loc, (size_t)have * 4, hipMemcpyDeviceToHost));
+    if (action) HIP_TRY(hipMemcpy(action, s->tr_action, (size_t)have, hipMemcpyDeviceToHost));
+    return (int)std::min<int64_t>(have, 0x7FFFFFFF);
 }
 
 int pt_sim_set_launch(pt_sim* s, int32_t knob, int32_t value) {

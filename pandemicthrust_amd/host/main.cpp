@@ -14,6 +14,8 @@
 //                           state to FILE (output_rn.csv and
 //                           infected_max.txt are then not written)
 //   --resume FILE           continue a run from FILE instead of seeding
+//   --validation            validation mode: debug.txt, debug_infected.csv,
+//                           debug_contacts.csv and debug_actions.csv
 #include <cstdlib>
 #include <cstring>
 
@@ -22,7 +24,8 @@
 static void usage() {
     fprintf(stderr,
             "usage: pandemicthrust [--output-dir DIR] [--device K] [--no-profile] [--poll-memory]\n"
-            "                      [--checkpoint FILE --checkpoint-day D] [--resume FILE] [input_dir]\n");
+            "                      [--checkpoint FILE --checkpoint-day D] [--resume FILE]\n"
+            "                      [--validation] [input_dir]\n");
     exit(2);
 }
 
@@ -42,6 +45,7 @@ int main(int argc, char** argv) {
         else if (!strcmp(a, "--checkpoint")) o.checkpoint_path = value();
         else if (!strcmp(a, "--checkpoint-day")) o.checkpoint_day = atoi(value());
         else if (!strcmp(a, "--resume")) o.resume_path = value();
+        else if (!strcmp(a, "--validation")) o.validation = true;
         else if (a[0] == '-' || have_dir) usage();
         else {
             o.input_dir = a;

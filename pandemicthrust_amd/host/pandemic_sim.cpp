@@ -206,9 +206,18 @@ PandemicSim::PandemicSim(const SimOptions& opts, ResourceLog* res)
         perror("Error opening profile_log.csv");
         exit(1);
     }
+    if (opt_.validation) {
+        val_.reset(new Validator);
+        if (!val_->open(out_dir_)) {
+            perror("Error opening validation files");
+            exit(1);
+        }
+    }
     loadParameters();
+    if (val_) val_->print("parameters loaded");
     const int32_t key[2] = {seeds_[0], seeds_[1]};
     if (pt_sim_create(&params_, key, opt_.device, &sim_) != PT_OK) fatal("create");
+    if (val_ && pt_sim_enable_trace(sim_, 1) != PT_OK) fatal("trace");
     double ph[PT_SETUP_PHASES] = {0};
     pt_sim_setup_times(sim_, ph, PT_SETUP_PHASES);
     // Constructor-level functions sit at stack depth 0 (PandemicSim.cu:121-127).
@@ -228,6 +237,7 @@ PandemicSim::~PandemicSim() {
     }
     if (f_stats_) fclose(f_stats_);
     prof_.close();
+    val_.reset();
     pt_sim_destroy(sim_);
 }
 
@@ -235,7 +245,9 @@ PandemicSim::~PandemicSim() {
 // loaded state replaces setup_initialInfected.
 void PandemicSim::setupSim() {
     const auto t0 = Clock::now();
+    if (val_) val_->print("setting up households");
     if (pt_sim_setup(sim_, &number_people_) != PT_OK) fatal("setup");
+    if (val_) val_->setup(sim_);  // debug_validatePeopleSetup (PandemicSim.cu:186-193)
     int64_t sc[8];
     if (pt_sim_get_scalars(sim_, sc) != PT_OK) fatal("setup");
     number_households_ = (int)sc[1];
@@ -268,6 +280,7 @@ void PandemicSim::setupSim() {
             fatal("initial infected");
     }
     const double seed_ms = ms_since(t_seed);
+    if (val_) val_->print("simulation setup complete");
     // must be done every simulation (PandemicSim.cu:195-196)
     res_->sample(0);
     double ph[PT_SETUP_PHASES] = {0};
@@ -366,7 +379,7 @@ void PandemicSim::runToCompletion() {
     if (pt_sim_get_scalars(sim_, sc) != PT_OK) fatal("run");
     const int first_day = (int)sc[7];  // rows already produced (a resumed run)
     if (opt_.profile && pt_sim_enable_timing(sim_, -1) != PT_OK) fatal("timing");
-    const bool stepwise = opt_.poll_memory || opt_.checkpoint_day >= 0;
+    const bool stepwise = opt_.poll_memory || opt_.checkpoint_day >= 0 || val_;
     if (stepwise) {
         int32_t c[16], inf = 0;
         for (int d = first_day; d < md; ++d) {
@@ -374,9 +387,11 @@ void PandemicSim::runToCompletion() {
                 stopped_ = true;
                 break;
             }
+            if (val_) val_->print("beginning day...");
             const int rc = pt_sim_step(sim_, c, &inf);
             if (rc < 0) fatal("step");
             if (inf == 0 || rc == 1) break;
+            if (val_) val_->day(sim_, d, inf);
             if (opt_.poll_memory) res_->sample(d);
         }
     } else {

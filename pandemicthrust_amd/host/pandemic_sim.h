@@ -7,17 +7,20 @@
 // reference does with its default switches (PandemicSim.h:23-45), it writes
 // profile_log.csv (profiler.cu), appends output_resource_log.csv
 // (resource_logging.cu:83-122) and, on request, output_mem.csv
-// (resource_logging.cu:31-81).  All device work goes through the C ABI of
+// (resource_logging.cu:31-81), and in validation mode debug.txt and the
+// debug CSVs (validation.h).  All device work goes through the C ABI of
 // include/pandemic/pt_abi.h.
 #pragma once
 
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "../../include/pandemic/pt_abi.h"
+#include "validation.h"
 
 struct SimOptions {
     std::string input_dir = ".";
@@ -28,6 +31,7 @@ struct SimOptions {
     std::string checkpoint_path;    // write a checkpoint here ...
     int checkpoint_day = -1;        // ... once this many days have run, then stop
     std::string resume_path;        // continue from this checkpoint instead of seeding
+    bool validation = false;        // SIM_VALIDATION (PandemicSim.h:50): debug.txt and the debug CSVs
 };
 
 // Memory samples and the run clock of resource_logging.cu.
@@ -95,4 +99,5 @@ private:
     float max_proportion_ = 0.f;
     int max_proportion_day_ = 0;
     bool stopped_ = false;
+    std::unique_ptr<Validator> val_;
 };
