@@ -193,40 +193,61 @@ __device__ __forceinline__ int32_t assign_workplace(const Model& M, uint32_t r, 
 }
 
 // ---------------- errand draws ----------------
+// The errand walks below run over rows kFirstErrandRow..kTypes-1 with every
+// step unrolled and predicated: each step subtracts the row's pdf only where
+// the reference's loop would continue, so the float operations and their
+// order are exactly the loop's, while the table reads are uniform (constant
+// row indices) instead of per-lane gathers inside a divergent loop.
+static_assert(kTypes - kFirstErrandRow == 5, "errand walks are unrolled over 5 rows");
+
+struct RowPick {
+    float frac;   // the walk's remaining uniform
+    float pdf;    // pdf of the row it stopped at
+    int count;    // locations of that row
+    int offset;   // first location of that row
+};
+
+// while (frac > pdf[row] && row < kTypes - 1) { frac -= pdf[row]; row++; }
+// from row kFirstErrandRow, over the pdf row table `pdf`.
+__device__ __forceinline__ RowPick walk_errand_rows(const Model& M, const float* pdf, float frac) {
+    RowPick r{frac, pdf[kFirstErrandRow], M.wp_count[kFirstErrandRow], M.wp_offset[kFirstErrandRow]};
+    bool go = true;
+#pragma unroll
+    for (int row = kFirstErrandRow; row < kTypes - 1; ++row) {
+        go = go && r.frac > pdf[row];
+        const float rest = r.frac - pdf[row];
+        r.frac = go ? rest : r.frac;
+        r.pdf = go ? pdf[row + 1] : r.pdf;
+        r.count = go ? M.wp_count[row + 1] : r.count;
+        r.offset = go ? M.wp_offset[row + 1] : r.offset;
+    }
+    return r;
+}
+
 // device_fishAfterschoolOrErrandDestination_weekday (PandemicSim.cu:2909-2947)
 __device__ __forceinline__ int32_t fish_weekday_dest(const Model& M, uint32_t r, bool adult) {
-    int bt = kTypeAfterschool;
     float frac = unit_float(r);
+    int type_count = M.wp_count[kTypeAfterschool];
+    int type_offset = M.wp_offset[kTypeAfterschool];
     if (adult) {
-        bt = kFirstErrandRow;
-        float row_pdf = M.wd_pdf[bt];
-        while (frac > row_pdf && bt < kTypes - 1) {
-            frac -= row_pdf;
-            bt++;
-            row_pdf = M.wd_pdf[bt];
-        }
-        frac = frac / row_pdf;
+        const RowPick p = walk_errand_rows(M, M.wd_pdf, frac);
+        frac = p.frac / p.pdf;
+        type_count = p.count;
+        type_offset = p.offset;
     }
-    const int type_count = M.wp_count[bt];
     uint32_t bn = (uint32_t)(frac * (float)type_count);
     if (bn >= (uint32_t)type_count) bn = (uint32_t)(type_count - 1);
-    bn += (uint32_t)M.wp_offset[bt];
+    bn += (uint32_t)type_offset;
     return (int32_t)bn;
 }
 
 // device_fishWeekendErrandDestination (PandemicSim.cu:1017-1044)
 __device__ __forceinline__ int32_t fish_weekend_dest(const Model& M, uint32_t r) {
-    float y = unit_float(r);
-    int row = kFirstErrandRow;
-    while (y > M.we_pdf[row] && row < kTypes - 1) {
-        y -= M.we_pdf[row];
-        row++;
-    }
-    const float frac = y / M.we_pdf[row];
-    const int type_count = M.wp_count[row];
-    int bn = (int)(frac * (float)type_count);
-    if (bn >= type_count) bn = type_count - 1;
-    return bn + M.wp_offset[row];
+    const RowPick p = walk_errand_rows(M, M.we_pdf, unit_float(r));
+    const float frac = p.frac / p.pdf;
+    int bn = (int)(frac * (float)p.count);
+    if (bn >= p.count) bn = p.count - 1;
+    return bn + p.offset;
 }
 
 // Weekday errand destinations of one person from its two draws
