@@ -235,10 +235,11 @@ int64_t pt_sim_trace_size(pt_sim* sim);
 int pt_sim_trace(pt_sim* sim, int32_t* infector, int32_t* victim, uint8_t* type, int32_t* loc, uint8_t* action,
                  int64_t n);
 
-/* Launch shapes.  They change only speed: results are identical for every
- * accepted value (the kernels stride over their work, and the radix sort
- * is stable at any tile size).  pandemicthrust_amd/autotune.py picks them
- * by measurement. */
+/* Launch shapes and table-strategy thresholds.  They change only speed:
+ * results are identical for every accepted value (the kernels stride over
+ * their work, the radix sort is stable at any tile size, and a table of the
+ * queried buckets answers every query a full table does).
+ * pandemicthrust_amd/autotune.py picks them by measurement. */
 enum pt_launch_knob {
     PT_LAUNCH_CONTACT_BLOCKS_PER_CU = 0, /* blocks per compute unit of the mark, contact and apply kernels
                                             (1..64, default 8) */
@@ -246,6 +247,10 @@ enum pt_launch_knob {
     PT_LAUNCH_SORT_WIDE_BITS,            /* radix passes of at least this many bits (at most 8) run 1024-thread
                                             blocks on 8192-pair tiles, narrower ones 512 on 4096 (1..9,
                                             default 8; 9 = never) */
+    PT_LAUNCH_MARK_LIMIT,                /* the day's queried buckets are marked (for a possible subset table)
+                                            while infected <= K * value / 100, K = buckets (0..1000, default 100) */
+    PT_LAUNCH_SUBSET_SHARE,              /* a marked day builds a subset table while marked buckets <
+                                            K * value / 100 (0..100, default 75) */
     PT_LAUNCH_KNOBS
 };
 int pt_sim_set_launch(pt_sim* sim, int32_t knob, int32_t value);

@@ -1,11 +1,15 @@
-"""Launch-shape autotuner for the daily loop and the replicate batch.
+"""Launch-shape and table-strategy autotuner for the daily loop and the
+replicate batch.
 
 The engines expose their launch shapes as settings (pt_launch_knob in
 include/pandemic/pt_abi.h): blocks per compute unit of the contact kernels
-and of the errand-key kernels, and the digit width from which radix passes
-run 1024-thread blocks.  They change only speed — every kernel strides over
-its work and the table sort is stable at any tile size — so the tuner may
-pick them by measurement alone.  It runs the workload once per candidate
+and of the errand-key kernels, the digit width from which radix passes run
+1024-thread blocks, and the two thresholds of the errand-table strategy (up
+to which infected count a day marks its queried buckets, and below which
+share of marked buckets it sorts only those).  They change only speed —
+every kernel strides over its work, the table sort is stable at any tile
+size, and a table of the queried buckets answers every query of the day —
+so the tuner may pick them by measurement alone.  It runs the workload once per candidate
 from the same snapshot (coordinate descent: one setting at a time, the
 others at their best so far) and keeps the fastest by the engine's own
 loop timer (HIP events around the enqueued days).
@@ -19,7 +23,11 @@ CANDIDATES = {
     "contact_blocks_per_cu": [2, 4, 8, 16, 32],
     "gen_blocks_per_cu": [4, 8, 16, 32],
     "sort_wide_bits": [6, 7, 8, 9],
+    "subset_share_pct": [60, 75, 90],
+    "mark_limit_pct": [20, 50, 100, 200],
 }
+SIM_KNOBS = ["contact_blocks_per_cu", "gen_blocks_per_cu", "sort_wide_bits", "subset_share_pct", "mark_limit_pct"]
+BATCH_KNOBS = ["contact_blocks_per_cu", "sort_wide_bits", "subset_share_pct", "mark_limit_pct"]
 
 
 # A candidate replaces the current value only when it is faster by more
@@ -59,7 +67,7 @@ def tune_sim(sim, repeats=1):
         return sim.last_run_ms()
 
     run()  # first-run costs (code objects, caches) out of the measurement
-    return tune(sim, run, ["contact_blocks_per_cu", "gen_blocks_per_cu", "sort_wide_bits"], repeats)
+    return tune(sim, run, SIM_KNOBS, repeats)
 
 
 def tune_batch(batch, repeats=1):
@@ -72,7 +80,7 @@ def tune_batch(batch, repeats=1):
         return batch.last_run_ms()
 
     run()
-    return tune(batch, run, ["contact_blocks_per_cu", "sort_wide_bits"], repeats)
+    return tune(batch, run, BATCH_KNOBS, repeats)
 
 
 def main(argv=None):

@@ -142,7 +142,7 @@ __global__ __launch_bounds__(kRepThreads) void kb_plan(const DayState* __restric
                                                        const uint32_t* __restrict__ blk_ninf,
                                                        uint32_t* __restrict__ blk_off, int32_t* __restrict__ rep_I,
                                                        int32_t* __restrict__ out_counts, int32_t* __restrict__ out_inf,
-                                                       int D, int table_mode) {
+                                                       int D, int table_mode, int mark_pct) {
     if (g->done) return;
     const int r = blockIdx.x;
     DayState* st = sts + r;
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(kRepThreads) void kb_plan(const DayState* __restric
             const Model& M = models[r];
             const bool weekend = (day % 7) >= 5;
             const int32_t K = weekend ? kWeHours * M.we_E : M.wd_K;
-            const bool mark = table_mode == TABLES_SUBSET || (table_mode == TABLES_AUTO && I <= K / 5);
+            const bool mark = table_mode == TABLES_SUBSET || (table_mode == TABLES_AUTO && I <= mark_limit(K, mark_pct));
             st->skip_mark = mark ? 0 : 1;
             st->skip_full = mark ? 1 : 0;
             const int64_t N = ri.N;
@@ -303,7 +303,8 @@ __global__ __launch_bounds__(256) void kb_mark(const DayState* __restrict__ g, c
 __global__ __launch_bounds__(kRepThreads) void kb_decide(const DayState* __restrict__ g, DayState* __restrict__ sts,
                                                          const RepInfo* __restrict__ reps,
                                                          const Model* __restrict__ models,
-                                                         const uint32_t* __restrict__ mark, int weekend, int table_mode) {
+                                                         const uint32_t* __restrict__ mark, int weekend, int table_mode,
+                                                         int share_pct) {
     if (g->done) return;
     const int r = blockIdx.x;
     DayState* st = sts + r;
@@ -323,7 +324,7 @@ __global__ __launch_bounds__(kRepThreads) void kb_decide(const DayState* __restr
         uint32_t m = 0;
         for (int i = 0; i < kRepThreads / 64; ++i) m += s_w[i];
         st->n_marked = m;
-        const bool subset = table_mode == TABLES_SUBSET || (table_mode == TABLES_AUTO && subset_pays(m, K));
+        const bool subset = table_mode == TABLES_SUBSET || (table_mode == TABLES_AUTO && subset_pays(m, K, share_pct));
         st->skip_full = subset ? 1 : 0;
     }
 }
@@ -503,7 +504,7 @@ struct pt_batch {
     TableScratch tables;
     int grid_contacts = 2048;
     int cus = 256;                                   // compute units of the device
-    int launch[PT_LAUNCH_KNOBS] = {8, 16, 8};        // pt_launch_knob values
+    int launch[PT_LAUNCH_KNOBS] = {8, 16, 8, 100, 75};  // pt_launch_knob values
     hipEvent_t run_begin = nullptr, run_end = nullptr;
     // The day step captured once per layout as HIP graphs (weekday,
     // weekend) and replayed for every day; the kernels take the day and
@@ -603,7 +604,8 @@ int enqueue_batch_day(pt_batch* b, int d) {
     kb_sweep<<<b->ntiles, kSweepThreads, 0, s>>>(g, b->d_states, b->d_reps, b->d_models, b->tile_rep, b->st_p, b->st_s, b->dy_p,
                                                  b->dy_s, b->blk_counts, b->blk_ninf, b->staging);
     kb_plan<<<R, kRepThreads, 0, s>>>(g, b->d_states, b->d_reps, b->d_models, b->blk_counts, b->blk_ninf, b->blk_off,
-                                      b->rep_I, b->d_counts, b->d_inf, b->D, b->table_mode);
+                                      b->rep_I, b->d_counts, b->d_inf, b->D, b->table_mode,
+                                      b->launch[PT_LAUNCH_MARK_LIMIT]);
     kb_start<<<1, 1024, 0, s>>>(g, b->rep_I, b->inf_start, R);
     kb_compact<<<b->ntiles, 256, 0, s>>>(g, b->d_states, b->tile_rep, b->staging, b->blk_ninf, b->blk_off,
                                          b->inf_start, b->infected);
@@ -618,7 +620,7 @@ int enqueue_batch_day(pt_batch* b, int d) {
         kb_mark<false><<<b->grid_contacts, 256, 0, s>>>(g, b->d_states, b->d_reps, b->d_models, b->inf_start, R,
                                                         b->infected, B, b->mark);
     kb_decide<<<R, kRepThreads, 0, s>>>(g, b->d_states, b->d_reps, b->d_models, b->mark, weekend ? 1 : 0,
-                                        b->table_mode);
+                                        b->table_mode, b->launch[PT_LAUNCH_SUBSET_SHARE]);
     const int nb = (int)(b->P / (weekend ? kSelAgentsWe : kSelAgentsWd));
     if (weekend)
         kb_select<kSelPerWe, true><<<nb, kSelThreads, 0, s>>>(g, b->d_states, b->d_reps, b->d_models, b->tile_rep,

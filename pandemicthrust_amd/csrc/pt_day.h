@@ -131,7 +131,12 @@ constexpr int kSelStageWd = kSelAgentsWd * 2;  // staged pairs per weekday selec
 constexpr int kSelStageWe = kSelAgentsWe * 3;  // staged pairs per weekend selection block
 
 // Whether a subset table pays off: fewer than 60% of the buckets queried.
-__host__ __device__ __forceinline__ bool subset_pays(int64_t marked, int64_t K) { return marked * 10 < K * 6; }
+__host__ __device__ __forceinline__ bool subset_pays(int64_t marked, int64_t K, int share_pct) {
+    return marked * 100 < K * share_pct;
+}
+
+// Infected count up to which a day marks its queried buckets (K * pct / 100).
+__host__ __device__ __forceinline__ int64_t mark_limit(int64_t K, int pct) { return K * pct / 100; }
 
 // ---------------- wave helpers ----------------
 
@@ -722,8 +727,10 @@ __device__ __forceinline__ void set_mark(uint32_t* __restrict__ mark, int32_t ke
 // gets one; PandemicSim.cu:1776-1813).
 __device__ __forceinline__ void mark_weekday(const Model& M, const AgentArrays& A, int day, uint64_t eoff, int32_t idx,
                                              uint32_t* __restrict__ mark, uint32_t kbias) {
+    // The contact kernels also draw for infectors that cannot infect when
+    // tracing (their contacts are traced), so their buckets are queried too.
     const Infectivity f = infectivity(M, A, idx, day);
-    if (!can_infect(M, f)) return;
+    if (!can_infect(M, f) && !A.trace_victim) return;
     const bool adult = A.age[idx] == kAgeAdult;
     const U4 ue = draw4(eoff + (uint64_t)(idx / 2), M.key0, M.key1);
     const int rs = 2 * (idx % 2);
@@ -741,7 +748,7 @@ __device__ __forceinline__ void mark_weekend(const Model& M, const AgentArrays& 
                                              uint64_t eoff, int64_t pos, int32_t idx, uint32_t* __restrict__ mark,
                                              uint32_t kbias) {
     const Infectivity f = infectivity(M, A, idx, day);
-    if (!can_infect(M, f)) return;
+    if (!can_infect(M, f) && !A.trace_victim) return;  // as mark_weekday
     const U4 u = draw4(cbase + (uint64_t)(6 * pos), M.key0, M.key1);
     const int prof = (int)(u.w[3] % 6u);
     int32_t hour[3], loc[3];

@@ -56,7 +56,7 @@ int fail(int code, const char* fmt, ...) {
 
 int check_launch(int32_t knob, int32_t value) {
     if (knob < 0 || knob >= PT_LAUNCH_KNOBS) return fail(PT_EINVAL, "launch setting %d unknown", knob);
-    const int lo[PT_LAUNCH_KNOBS] = {1, 1, 1}, hi[PT_LAUNCH_KNOBS] = {64, 64, 9};
+    const int lo[PT_LAUNCH_KNOBS] = {1, 1, 1, 0, 0}, hi[PT_LAUNCH_KNOBS] = {64, 64, 9, 1000, 100};
     if (value < lo[knob] || value > hi[knob])
         return fail(PT_EINVAL, "launch setting %d: value %d outside [%d, %d]", knob, value, lo[knob], hi[knob]);
     return PT_OK;
@@ -360,7 +360,7 @@ __global__ void k_clear_marks(const DayState* __restrict__ st, uint32_t* __restr
 
 // Chooses today's table path from the number of queried buckets.
 __global__ __launch_bounds__(kPlanThreads) void k_decide(DayState* __restrict__ st, const uint32_t* __restrict__ mark,
-                                                         int64_t nwords, int32_t K, int forced) {
+                                                         int64_t nwords, int32_t K, int forced, int share_pct) {
     if (st->done || st->skip_mark) return;
     __shared__ uint32_t s_w[kPlanThreads / 64];
     uint32_t c = 0;
@@ -372,7 +372,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_decide(DayState* __restrict__ 
         uint32_t m = 0;
         for (int w = 0; w < kPlanThreads / 64; ++w) m += s_w[w];
         st->n_marked = m;
-        const bool subset = forced == TABLES_SUBSET || (forced == TABLES_AUTO && (int64_t)m * 10 < (int64_t)K * 6);
+        const bool subset = forced == TABLES_SUBSET || (forced == TABLES_AUTO && subset_pays(m, K, share_pct));
         st->skip_subset = subset ? 0 : 1;
         st->skip_full = subset ? 1 : 0;
     }
@@ -834,7 +834,7 @@ struct pt_sim {
     int32_t *tr_victim = nullptr, *tr_loc = nullptr;
     uint8_t *tr_type = nullptr, *tr_attempt = nullptr, *tr_action = nullptr;
     int cus = 128;                                   // compute units of the device
-    int launch[PT_LAUNCH_KNOBS] = {8, 16, 8};        // pt_launch_knob values
+    int launch[PT_LAUNCH_KNOBS] = {8, 16, 8, 100, 75};  // pt_launch_knob values
 };
 
 namespace {
@@ -943,9 +943,11 @@ int enqueue_day(pt_sim* s, int d) {
     launch(tm, KC_SWEEP, k_sweep, dim3(s->nblk_sweep), dim3(kSweepThreads), stc, (const Model*)s->d_model, s->N, st,
            s->st_p, s->st_s,
            (const uint8_t*)s->dy_p, (const uint8_t*)s->dy_s, s->blk_counts, s->blk_ninf, s->staging);
-    // Marking pays off while the expected queried fraction stays below about
-    // 0.4 (two queried buckets per infector): I <= K / 5.
-    const int32_t lim_wd = s->model.wd_K / 5, lim_we = kWeHours * s->model.we_E / 5;
+    // Marking pays off while the queried share of the buckets is likely to
+    // stay below the subset threshold: I <= K * mark_limit / 100.
+    const int mpct = s->launch[PT_LAUNCH_MARK_LIMIT];
+    const int32_t lim_wd = (int32_t)std::min<int64_t>(mark_limit(s->model.wd_K, mpct), INT32_MAX);
+    const int32_t lim_we = (int32_t)std::min<int64_t>(mark_limit((int64_t)kWeHours * s->model.we_E, mpct), INT32_MAX);
     launch(tm, KC_PLAN, k_plan, dim3(1), dim3(kPlanThreads), stc, st, s->nblk_sweep, (const uint32_t*)s->blk_counts,
            (const uint32_t*)s->blk_ninf, s->blk_off, s->d_counts, s->d_inf, s->N, lim_wd, lim_we, s->table_mode);
     launch(tm, KC_COMPACT, k_compact, dim3(s->nblk_sweep), dim3(256), stc, (const DayState*)st,
@@ -967,7 +969,7 @@ int enqueue_day(pt_sim* s, int d) {
         launch(tm, KC_MARK, k_mark_weekday, dim3(s->grid_contacts), dim3(256), stc, M, (const DayState*)st, infected, A,
                s->mark);
     launch(tm, KC_MARK, k_decide, dim3(1), dim3(kPlanThreads), stc, st, (const uint32_t*)s->mark, nwords, K,
-           s->table_mode);
+           s->table_mode, s->launch[PT_LAUNCH_SUBSET_SHARE]);
     // Full path: every agent's keys, implicit ids.
     if (weekend)
         launch(tm, KC_GEN_WEEKEND, k_gen_weekend, dim3(s->grid_gen), dim3(256), stc, M, st, s->keys);

@@ -1,5 +1,6 @@
-"""Launch shapes (pt_launch_knob) change speed only: every setting the
-autotuner may pick gives bit-identical days, agent state and tables."""
+"""Launch shapes and table-strategy thresholds (pt_launch_knob) change speed
+only: every setting the autotuner may pick gives bit-identical days, agent
+state and tables."""
 import numpy as np
 import pytest
 
@@ -8,6 +9,8 @@ from pandemicthrust_amd import PandemicBatch, PandemicSim, Params, PTError, auto
 pytestmark = pytest.mark.gpu
 
 ROW = ["40", "1.8", "1.3", "10", "10", "0.0425", "0.0425", "1", "0"]
+DEFAULTS = {"contact_blocks_per_cu": 8, "gen_blocks_per_cu": 16, "sort_wide_bits": 8, "mark_limit_pct": 100,
+            "subset_share_pct": 75}
 ROWS = [(["30", "2.0", "1.4", "10", "10", "0.03", "0.03", "1", "0"], 1),
         (["35", "1.7", "1.3", "20", "5", "0.05", "0.04", "0.9", "0.2"], 2),
         (["25", "2.4", "1.2", "8", "12", "0.02", "0.025", "1", "0"], 3)]
@@ -30,7 +33,10 @@ def test_sim_launch_settings_do_not_change_results(mode):
     ref = _sim_result({}, mode)
     for settings in ({"contact_blocks_per_cu": 1, "gen_blocks_per_cu": 1, "sort_wide_bits": 9},
                      {"contact_blocks_per_cu": 32, "gen_blocks_per_cu": 64, "sort_wide_bits": 1},
-                     {"contact_blocks_per_cu": 3, "gen_blocks_per_cu": 7, "sort_wide_bits": 6}):
+                     {"contact_blocks_per_cu": 3, "gen_blocks_per_cu": 7, "sort_wide_bits": 6},
+                     {"mark_limit_pct": 0, "subset_share_pct": 0},          # never a subset table
+                     {"mark_limit_pct": 1000, "subset_share_pct": 100},     # a subset table on nearly every day
+                     {"mark_limit_pct": 50, "subset_share_pct": 90}):
         got = _sim_result(settings, mode)
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), settings
         for a, b in zip(got[2], ref[2]):
@@ -42,7 +48,8 @@ def test_batch_launch_settings_do_not_change_results():
     b.setup()
     b.snapshot()
     ref = b.run()
-    for settings in ({"contact_blocks_per_cu": 1, "sort_wide_bits": 9}, {"contact_blocks_per_cu": 32, "sort_wide_bits": 2}):
+    for settings in ({"contact_blocks_per_cu": 1, "sort_wide_bits": 9}, {"contact_blocks_per_cu": 32, "sort_wide_bits": 2},
+                     {"mark_limit_pct": 1000, "subset_share_pct": 100}, {"mark_limit_pct": 0, "subset_share_pct": 0}):
         for k, v in settings.items():
             b.set_launch(k, v)
         b.restore()
@@ -54,11 +61,15 @@ def test_batch_launch_settings_do_not_change_results():
 
 def test_launch_settings_are_checked():
     s = PandemicSim(Params.from_row(ROW), 3)
-    assert s.launch() == {"contact_blocks_per_cu": 8, "gen_blocks_per_cu": 16, "sort_wide_bits": 8}
+    assert s.launch() == DEFAULTS
     with pytest.raises(PTError, match="outside"):
         s.set_launch("sort_wide_bits", 10)
     with pytest.raises(PTError, match="outside"):
         s.set_launch("contact_blocks_per_cu", 0)
+    with pytest.raises(PTError, match="outside"):
+        s.set_launch("mark_limit_pct", 1001)
+    with pytest.raises(PTError, match="outside"):
+        s.set_launch("subset_share_pct", 101)
     s.close()
 
 
@@ -67,11 +78,11 @@ def test_autotuner_keeps_results():
     s.setup_sim()
     ref_c, ref_i = None, None
     best, log = autotune.tune_sim(s)
-    assert set(best) == {"contact_blocks_per_cu", "gen_blocks_per_cu", "sort_wide_bits"}
+    assert set(best) == set(DEFAULTS) == set(autotune.SIM_KNOBS)
     assert len(log) == sum(len(autotune.CANDIDATES[k]) for k in best)
     for k, v in best.items():         # the default unless a candidate was clearly faster
         ms = {val: t for name, val, t in log if name == k}
-        assert v == {"contact_blocks_per_cu": 8, "gen_blocks_per_cu": 16, "sort_wide_bits": 8}[k] or \
+        assert v == DEFAULTS[k] or \
             ms[v] == min(ms.values()), (k, v, ms)
     assert s.launch() == best
     s.restore()

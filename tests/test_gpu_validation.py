@@ -54,12 +54,15 @@ def _check_locations(t, N, W, hh, wp, er, weekend):
             assert row[inf[i]] == loc[i] and row[vic[i]] == loc[i], i
 
 
-@pytest.mark.parametrize("row,seed", [
-    (["21", "1.8", "1.3", "10", "10", "0.0425", "0.0425", "1", "0"], 1),
-    (["16", "2.4", "1.6", "30", "20", "0.03", "0.06", "0.8", "0.4"], 5),
+@pytest.mark.parametrize("row,seed,mode", [
+    (["21", "1.8", "1.3", "10", "10", "0.0425", "0.0425", "1", "0"], 1, 0),
+    (["21", "1.8", "1.3", "10", "10", "0.0425", "0.0425", "1", "0"], 1, 1),   # full tables every day
+    (["21", "1.8", "1.3", "10", "10", "0.0425", "0.0425", "1", "0"], 1, 2),   # queried buckets only, every day
+    (["16", "2.4", "1.6", "30", "20", "0.03", "0.06", "0.8", "0.4"], 5, 0),
 ])
-def test_trace_matches_oracle_with_consistent_locations(row, seed):
+def test_trace_matches_oracle_with_consistent_locations(row, seed, mode):
     gpu = PandemicSim(Params.from_row(row), seed)
+    gpu.set_table_mode(mode)
     gpu.enable_trace()
     N = gpu.setup_sim()
     ref = O.OracleSim(O.params_from_row(row), seed)
