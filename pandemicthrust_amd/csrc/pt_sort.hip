@@ -384,12 +384,32 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_parts(uint32_t* part, int
     }
 }
 
-// Scan, phase 3: exclusive scan inside each segment plus its offset.
+// Scan, phase 3: exclusive scan inside each segment plus its offset.  With
+// SUM_PARTS the offset is summed here from the raw segment sums (at most
+// kScanFewParts segments), so phase 2 is not launched.
+constexpr int kScanFewParts = 64;
+
+__device__ __forceinline__ uint32_t wave_total(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <bool SUM_PARTS>
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t* __restrict__ in,
                                                             uint32_t* __restrict__ out, int64_t n,
                                                             const uint32_t* __restrict__ part, const int32_t* done) {
     if (is_done(done)) return;
     __shared__ uint32_t s_w[kScanThreads / 64];
+    __shared__ uint32_t s_off;
+    if (SUM_PARTS) {
+        if (threadIdx.x < 64) {
+            uint32_t v = (int)threadIdx.x < (int)blockIdx.x ? part[threadIdx.x] : 0u;
+            v = wave_total(v);
+            if (threadIdx.x == 0) s_off = v;
+        }
+        __syncthreads();
+    }
     const int64_t base = (int64_t)blockIdx.x * kScanSeg + (int64_t)threadIdx.x * kScanItems;
     uint32_t x[kScanItems];
     uint32_t sum = 0;
@@ -399,7 +419,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t* __re
         x[i] = j < n ? in[j] : 0u;
         sum += x[i];
     }
-    uint32_t run = block_excl_scan<kScanThreads>(sum, s_w, nullptr) + part[blockIdx.x];
+    uint32_t run = block_excl_scan<kScanThreads>(sum, s_w, nullptr) + (SUM_PARTS ? s_off : part[blockIdx.x]);
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
         const int64_t j = base + i;
@@ -542,8 +562,14 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint
     if (n <= 0) return hipSuccess;
     const int nparts = (int)((n + kScanSeg - 1) / kScanSeg);
     launch(tm, cls, k_scan_reduce, dim3(nparts), dim3(kScanThreads), stream, in, n, part, done);
-    launch(tm, cls, k_scan_parts, dim3(1), dim3(kScanThreads), stream, part, nparts, done);
-    launch(tm, cls, k_scan_down, dim3(nparts), dim3(kScanThreads), stream, in, out, n, (const uint32_t*)part, done);
+    if (nparts <= kScanFewParts) {
+        launch(tm, cls, k_scan_down<true>, dim3(nparts), dim3(kScanThreads), stream, in, out, n,
+               (const uint32_t*)part, done);
+    } else {
+        launch(tm, cls, k_scan_parts, dim3(1), dim3(kScanThreads), stream, part, nparts, done);
+        launch(tm, cls, k_scan_down<false>, dim3(nparts), dim3(kScanThreads), stream, in, out, n,
+               (const uint32_t*)part, done);
+    }
     return hipGetLastError();
 }
 
