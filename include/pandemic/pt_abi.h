@@ -251,6 +251,8 @@ enum pt_launch_knob {
                                             while infected <= K * value / 100, K = buckets (0..1000, default 100) */
     PT_LAUNCH_SUBSET_SHARE,              /* a marked day builds a subset table while marked buckets <
                                             K * value / 100 (0..100, default 75) */
+    PT_LAUNCH_SORT_MAX_BITS,             /* widest digit of a radix pass: K buckets take ceil(log2 K / value)
+                                            passes (4..11; default 8, 9 in a replicate batch) */
     PT_LAUNCH_KNOBS
 };
 int pt_sim_set_launch(pt_sim* sim, int32_t knob, int32_t value);

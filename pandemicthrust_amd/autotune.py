@@ -3,12 +3,12 @@ replicate batch.
 
 The engines expose their launch shapes as settings (pt_launch_knob in
 include/pandemic/pt_abi.h): blocks per compute unit of the contact kernels
-and of the errand-key kernels, the digit width from which radix passes run
-1024-thread blocks, and the two thresholds of the errand-table strategy (up
+and of the errand-key kernels, the widest digit of a radix pass, the digit
+width from which radix passes run 1024-thread blocks, and the two thresholds of the errand-table strategy (up
 to which infected count a day marks its queried buckets, and below which
 share of marked buckets it sorts only those).  They change only speed —
 every kernel strides over its work, the table sort is stable at any tile
-size, and a table of the queried buckets answers every query of the day —
+size and digit width, and a table of the queried buckets answers every query of the day —
 so the tuner may pick them by measurement alone.  It runs the workload once per candidate
 from the same snapshot (coordinate descent: one setting at a time, the
 others at their best so far) and keeps the fastest by the engine's own
@@ -25,9 +25,11 @@ CANDIDATES = {
     "sort_wide_bits": [6, 7, 8, 9],
     "subset_share_pct": [60, 75, 90],
     "mark_limit_pct": [20, 50, 100, 200],
+    "sort_max_bits": [8, 9, 10],
 }
-SIM_KNOBS = ["contact_blocks_per_cu", "gen_blocks_per_cu", "sort_wide_bits", "subset_share_pct", "mark_limit_pct"]
-BATCH_KNOBS = ["contact_blocks_per_cu", "sort_wide_bits", "subset_share_pct", "mark_limit_pct"]
+SIM_KNOBS = ["contact_blocks_per_cu", "gen_blocks_per_cu", "sort_max_bits", "sort_wide_bits", "subset_share_pct",
+             "mark_limit_pct"]
+BATCH_KNOBS = ["contact_blocks_per_cu", "sort_max_bits", "sort_wide_bits", "subset_share_pct", "mark_limit_pct"]
 
 
 # A candidate replaces the current value only when it is faster by more

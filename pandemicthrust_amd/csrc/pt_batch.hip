@@ -504,7 +504,7 @@ struct pt_batch {
     TableScratch tables;
     int grid_contacts = 2048;
     int cus = 256;                                   // compute units of the device
-    int launch[PT_LAUNCH_KNOBS] = {8, 16, 8, 100, 75};  // pt_launch_knob values
+    int launch[PT_LAUNCH_KNOBS] = {8, 16, 8, 100, 75, 9};  // pt_launch_knob values
     hipEvent_t run_begin = nullptr, run_end = nullptr;
     // The day step captured once per layout as HIP graphs (weekday,
     // weekend) and replayed for every day; the kernels take the day and
@@ -835,6 +835,7 @@ int pt_batch_setup(pt_batch* b, int64_t* n_people_out) {
         return fail(PT_ENOMEM, "table scratch allocation failed");
     }
     b->tables.wide_bits = b->launch[PT_LAUNCH_SORT_WIDE_BITS];
+    b->tables.max_bits = b->launch[PT_LAUNCH_SORT_MAX_BITS];
     HIP_TRY(hipMemcpyAsync(b->tile_rep, trep.data(), sizeof(int32_t) * tiles, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(b->d_models, b->models.data(), sizeof(Model) * R, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(b->d_reps, b->reps.data(), sizeof(RepInfo) * R, hipMemcpyHostToDevice, st));
@@ -1030,6 +1031,7 @@ int pt_batch_set_launch(pt_batch* b, int32_t knob, int32_t value) {
     b->launch[knob] = value;
     b->grid_contacts = b->cus * b->launch[PT_LAUNCH_CONTACT_BLOCKS_PER_CU];
     b->tables.wide_bits = b->launch[PT_LAUNCH_SORT_WIDE_BITS];
+    b->tables.max_bits = b->launch[PT_LAUNCH_SORT_MAX_BITS];
     drop_day_graphs(b);  // captured with the old launch shapes
     return PT_OK;
 }

@@ -56,7 +56,7 @@ int fail(int code, const char* fmt, ...) {
 
 int check_launch(int32_t knob, int32_t value) {
     if (knob < 0 || knob >= PT_LAUNCH_KNOBS) return fail(PT_EINVAL, "launch setting %d unknown", knob);
-    const int lo[PT_LAUNCH_KNOBS] = {1, 1, 1, 0, 0}, hi[PT_LAUNCH_KNOBS] = {64, 64, 9, 1000, 100};
+    const int lo[PT_LAUNCH_KNOBS] = {1, 1, 1, 0, 0, 4}, hi[PT_LAUNCH_KNOBS] = {64, 64, 9, 1000, 100, 11};
     if (value < lo[knob] || value > hi[knob])
         return fail(PT_EINVAL, "launch setting %d: value %d outside [%d, %d]", knob, value, lo[knob], hi[knob]);
     return PT_OK;
@@ -834,7 +834,7 @@ struct pt_sim {
     int32_t *tr_victim = nullptr, *tr_loc = nullptr;
     uint8_t *tr_type = nullptr, *tr_attempt = nullptr, *tr_action = nullptr;
     int cus = 128;                                   // compute units of the device
-    int launch[PT_LAUNCH_KNOBS] = {8, 16, 8, 100, 75};  // pt_launch_knob values
+    int launch[PT_LAUNCH_KNOBS] = {8, 16, 8, 100, 75, 8};  // pt_launch_knob values
 };
 
 namespace {
@@ -1218,6 +1218,7 @@ int pt_sim_setup(pt_sim* s, int64_t* n_people_out) {
         return fail(PT_ENOMEM, "table scratch allocation failed");
     }
     s->tables.wide_bits = s->launch[PT_LAUNCH_SORT_WIDE_BITS];
+    s->tables.max_bits = s->launch[PT_LAUNCH_SORT_MAX_BITS];
     if (s->trace_on && alloc_trace(s)) {
         free_population(s);
         return fail(PT_ENOMEM, "trace allocation failed");
@@ -1473,8 +1474,8 @@ int pt_sim_get_layout(pt_sim* s, int64_t out[8]) {
     out[3] = m.we_E;
     out[4] = m.wd_K;
     out[5] = (int64_t)kWeHours * m.we_E;
-    out[6] = radix_passes((int32_t)out[4]);
-    out[7] = radix_passes((int32_t)out[5]);
+    out[6] = radix_passes((int32_t)out[4], s->tables.max_bits);
+    out[7] = radix_passes((int32_t)out[5], s->tables.max_bits);
     return PT_OK;
 }
 
@@ -1758,6 +1759,7 @@ int pt_sim_set_launch(pt_sim* s, int32_t knob, int32_t value) {
     s->grid_contacts = s->cus * s->launch[PT_LAUNCH_CONTACT_BLOCKS_PER_CU];
     s->grid_gen = s->cus * s->launch[PT_LAUNCH_GEN_BLOCKS_PER_CU];
     s->tables.wide_bits = s->launch[PT_LAUNCH_SORT_WIDE_BITS];
+    s->tables.max_bits = s->launch[PT_LAUNCH_SORT_MAX_BITS];
     return PT_OK;
 }
 

@@ -88,6 +88,9 @@ struct TableScratch {
     // blocks on 8192-pair tiles, narrower ones 512-thread blocks on
     // 4096-pair tiles; 9 keeps every pass at 512 threads.
     int wide_bits = 8;
+    // Widest digit of a radix pass (4..11): a table of K buckets takes
+    // ceil(log2 K / max_bits) passes.
+    int max_bits = 8;
 };
 
 int table_scratch_alloc(TableScratch& s, int64_t n_cap);
@@ -106,11 +109,8 @@ hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int
                        int32_t K, IdMode mode, int32_t N, int32_t* people_out, int32_t* offsets_out,
                        const int32_t* done, hipStream_t stream, LaunchTimer* tm, uint32_t drop_key = kNoDrop);
 
-// Number of radix passes build_table uses for K buckets.
-int radix_passes(int32_t K);
-
-// Widest digit (bits) of one radix pass, 4..11 (default 8).
-void set_radix_max_bits(int bits);
+// Number of radix passes build_table uses for K buckets at max_bits per pass.
+int radix_passes(int32_t K, int max_bits);
 
 // Exclusive scan of n uint32 values (in and out may alias); part needs
 // ceil(n / 8192) + 1 entries.

@@ -37,7 +37,6 @@ constexpr int max_blocks() {
     return T >= 1024 ? 256 : 1024;
 }
 
-int g_max_bits = 8;  // widest digit per pass (see set_radix_max_bits)
 
 __device__ __forceinline__ int32_t implicit_id(int64_t j, int mode, int32_t N) {
     if (mode == ID_HOUR_MAJOR) return (int32_t)(j < N ? j : j - N);
@@ -525,12 +524,13 @@ int key_bits(int32_t K) {
 
 }  // namespace
 
-void set_radix_max_bits(int bits) { g_max_bits = std::min(std::max(bits, 4), kMaxBitsSupported); }
-
-int radix_passes(int32_t K) { return (key_bits(K) + g_max_bits - 1) / g_max_bits; }
+int radix_passes(int32_t K, int max_bits) {
+    const int mb = std::min(std::max(max_bits, 4), kMaxBitsSupported);
+    return (key_bits(K) + mb - 1) / mb;
+}
 
 int table_scratch_alloc(TableScratch& s, int64_t n_cap) {
-    const int wide_bits = s.wide_bits;  // a launch setting, kept across reallocation
+    const int wide_bits = s.wide_bits, max_bits = s.max_bits;  // launch settings, kept across reallocation
     table_scratch_free(s);
     const int64_t n = std::max<int64_t>(n_cap, 1);
     const int64_t hist_n = ((int64_t)1 << kMaxBitsSupported) * kHistBlocks + 1;
@@ -544,6 +544,7 @@ int table_scratch_alloc(TableScratch& s, int64_t n_cap) {
     if (hipMalloc(&s.scan_part, (size_t)parts * sizeof(uint32_t)) != hipSuccess) return -1;
     s.n_cap = n_cap;
     s.wide_bits = wide_bits;
+    s.max_bits = max_bits;
     return 0;
 }
 
@@ -580,7 +581,7 @@ hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int
     if (drop_key != kNoDrop && (n_dev || drop_key < (uint32_t)K - 1u || drop_key >= (uint32_t)K))
         return hipErrorInvalidValue;  // only the last bucket of a host-sized table can be dropped
     const int bits = key_bits(K);
-    const int passes = radix_passes(K);
+    const int passes = radix_passes(K, s.max_bits);
     // Split the bits as evenly as possible over the passes.
     int pass_bits[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int p = 0; p < passes; ++p) pass_bits[p] = bits / passes + (p < bits % passes ? 1 : 0);

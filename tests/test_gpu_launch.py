@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 
 ROW = ["40", "1.8", "1.3", "10", "10", "0.0425", "0.0425", "1", "0"]
 DEFAULTS = {"contact_blocks_per_cu": 8, "gen_blocks_per_cu": 16, "sort_wide_bits": 8, "mark_limit_pct": 100,
-            "subset_share_pct": 75}
+            "subset_share_pct": 75, "sort_max_bits": 8}
 ROWS = [(["30", "2.0", "1.4", "10", "10", "0.03", "0.03", "1", "0"], 1),
         (["35", "1.7", "1.3", "20", "5", "0.05", "0.04", "0.9", "0.2"], 2),
         (["25", "2.4", "1.2", "8", "12", "0.02", "0.025", "1", "0"], 3)]
@@ -36,7 +36,8 @@ def test_sim_launch_settings_do_not_change_results(mode):
                      {"contact_blocks_per_cu": 3, "gen_blocks_per_cu": 7, "sort_wide_bits": 6},
                      {"mark_limit_pct": 0, "subset_share_pct": 0},          # never a subset table
                      {"mark_limit_pct": 1000, "subset_share_pct": 100},     # a subset table on nearly every day
-                     {"mark_limit_pct": 50, "subset_share_pct": 90}):
+                     {"mark_limit_pct": 50, "subset_share_pct": 90},
+                     {"sort_max_bits": 4}, {"sort_max_bits": 11, "sort_wide_bits": 1}):
         got = _sim_result(settings, mode)
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), settings
         for a, b in zip(got[2], ref[2]):
@@ -49,7 +50,8 @@ def test_batch_launch_settings_do_not_change_results():
     b.snapshot()
     ref = b.run()
     for settings in ({"contact_blocks_per_cu": 1, "sort_wide_bits": 9}, {"contact_blocks_per_cu": 32, "sort_wide_bits": 2},
-                     {"mark_limit_pct": 1000, "subset_share_pct": 100}, {"mark_limit_pct": 0, "subset_share_pct": 0}):
+                     {"mark_limit_pct": 1000, "subset_share_pct": 100}, {"mark_limit_pct": 0, "subset_share_pct": 0},
+                     {"sort_max_bits": 4}, {"sort_max_bits": 11}):
         for k, v in settings.items():
             b.set_launch(k, v)
         b.restore()
@@ -70,6 +72,8 @@ def test_launch_settings_are_checked():
         s.set_launch("mark_limit_pct", 1001)
     with pytest.raises(PTError, match="outside"):
         s.set_launch("subset_share_pct", 101)
+    with pytest.raises(PTError, match="outside"):
+        s.set_launch("sort_max_bits", 12)
     s.close()
 
 
