@@ -28,7 +28,8 @@ engine at 1M agents x 120 days (surrogate training data generation).
 surrogate: a 5-member MLP ensemble trained with the BF16 MFMA kernels on
 simulated scenario ensembles (100k agents); training TFLOP/s against the
 MFMA peak, held-out accuracy, and a screening sweep (surrogate inference,
-re-simulation of flagged vectors) against simulating every vector.
+re-simulation of flagged vectors) against simulating every vector, both
+plainly (one run at a time, timed on a sample) and on the batch engine.
 
 cpu_baseline: the CPU oracle (a sequential port of the reference semantics)
 on one host core, timed on a bounded sample of the same workload: setup plus
@@ -278,7 +279,22 @@ def surrogate_bench(n_train, n_screen):
     pf = sc["mean"][chk].reshape(len(chk), 2, days, 3)
     fd, fh, fa = curve_stats(pf[:, 0, :, 0])
     rd, rh, ra = curve_stats(Yf[:, 0, :, 0])
-    plain_s = n_screen * reps / itr["runs_per_s"]
+    batched_s = n_screen * reps / itr["runs_per_s"]
+    # Plain simulation, one run at a time as in the replicate leg: a sample of
+    # the screened scenarios, every replicate its own handle (create, set up,
+    # run to the end).
+    from pandemicthrust_amd import PandemicSim
+    n_plain = min(8, n_screen)
+    t0 = time.perf_counter()
+    for i in range(n_plain):
+        p, iv = E.scenario_inputs(Xs[i], agents, days)
+        for r in range(reps):
+            sim = PandemicSim(p, 3 * 10 ** 6 + i * reps + r)
+            sim.set_interventions(iv)
+            sim.setup_sim()
+            sim.run()
+            sim.close()
+    plain_s = n_screen * (time.perf_counter() - t0) / n_plain
     sur.close()
     print("surrogate: %d steps in %.2f s (%.1f TFLOP/s), held-out rmse %.3f" % (steps, train_ms / 1e3, tflops, rmse),
           file=sys.stderr, flush=True)
@@ -298,7 +314,11 @@ def surrogate_bench(n_train, n_screen):
                           "flagged_attack_rel_err": round(float(np.median(np.abs(fa - ra) / np.maximum(ra, 1.0))), 4)
                           if len(chk) else None,
                           "end_to_end_per_s": round(n_screen / (infer_s + resim_s), 1),
-                          "plain_simulation_per_s": round(n_screen / plain_s, 1)}}
+                          "plain_simulation_per_s": round(n_screen / plain_s, 1),
+                          "plain_sample": "%d scenarios x %d replicates, one run at a time" % (n_plain, reps),
+                          "batched_simulation_per_s": round(n_screen / batched_s, 1),
+                          "speedup_vs_plain": round(plain_s / (infer_s + resim_s), 1),
+                          "speedup_vs_batched": round(batched_s / (infer_s + resim_s), 1)}}
 
 
 def main():
