@@ -354,6 +354,16 @@ __device__ __forceinline__ bool select_block(const Model& M, uint64_t eoff, cons
             }
         }
     } else {
+        // The thread's ages in one load when its agents are all inside the
+        // population (a0 is a multiple of PER; age is PER-byte aligned).
+        uint8_t ag[PER];
+        if (PER == 8 && a0 + PER <= N) {
+            const uint2 v = *reinterpret_cast<const uint2*>(age + a0);
+            memcpy(ag, &v, 8);
+        } else {
+#pragma unroll
+            for (int q = 0; q < PER; ++q) ag[q] = a0 + q < N ? age[a0 + q] : 0;
+        }
 #pragma unroll
         for (int q = 0; q < PER / 2; ++q) {
             const int64_t g = a0 / 2 + q;
@@ -365,7 +375,7 @@ __device__ __forceinline__ bool select_block(const Model& M, uint64_t eoff, cons
                     const int64_t p = 2 * g + k;
                     if (p < N) {
                         int32_t d1, d2;
-                        weekday_dests(M, u.w[2 * k], u.w[2 * k + 1], age[p] == kAgeAdult, d1, d2);
+                        weekday_dests(M, u.w[2 * k], u.w[2 * k + 1], ag[2 * q + k] == kAgeAdult, d1, d2);
                         const int i = 4 * q + 2 * k;
                         bad |= !weekday_keys(M, d1, d2, key[i], key[i + 1]);
                         key[i] += kbias;
