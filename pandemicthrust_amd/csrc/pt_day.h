@@ -155,6 +155,34 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
+// Sums over the 64 lanes of eight byte counters packed in v (bin b in byte
+// b; at most 16 per lane): three butterfly steps on the packed words (a byte
+// reaches at most 128), then three on 16-bit fields.  out[b] = the total of
+// bin b, in every lane.
+__device__ __forceinline__ void wave_sum_bins8(uint64_t v, uint32_t (&out)[8]) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+#pragma unroll
+    for (int d = 1; d < 8; d <<= 1) {
+        lo += __shfl_xor(lo, d, 64);
+        hi += __shfl_xor(hi, d, 64);
+    }
+    uint32_t w[4] = {lo & 0x00FF00FFu, (lo >> 8) & 0x00FF00FFu, hi & 0x00FF00FFu, (hi >> 8) & 0x00FF00FFu};
+#pragma unroll
+    for (int d = 8; d < 64; d <<= 1) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) w[j] += __shfl_xor(w[j], d, 64);
+    }
+    // w[0]: bins 0 and 2, w[1]: 1 and 3, w[2]: 4 and 6, w[3]: 5 and 7
+    out[0] = w[0] & 0xFFFFu;
+    out[2] = w[0] >> 16;
+    out[1] = w[1] & 0xFFFFu;
+    out[3] = w[1] >> 16;
+    out[4] = w[2] & 0xFFFFu;
+    out[6] = w[2] >> 16;
+    out[5] = w[3] & 0xFFFFu;
+    out[7] = w[3] >> 16;
+}
+
 // All 64 lanes must call: reserves n_mine slots per lane in a list whose
 // length lives at *counter; returns the lane's first slot.
 __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, uint32_t n_mine, int lane) {
@@ -273,13 +301,16 @@ __device__ __forceinline__ void sweep_block(const Model& M, int64_t N, int day, 
     }
     __shared__ uint32_t s_cnt[kSweepThreads / 64][16];
     __shared__ uint32_t s_inf[kSweepThreads / 64];
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-        const uint32_t vp = wave_sum((uint32_t)((cp >> (8 * b)) & 0xFF));
-        const uint32_t vs = wave_sum((uint32_t)((cs >> (8 * b)) & 0xFF));
+    {
+        uint32_t vp[8], vs[8];
+        wave_sum_bins8(cp, vp);
+        wave_sum_bins8(cs, vs);
         if (lane == 0) {
-            s_cnt[wave][b] = vp;
-            s_cnt[wave][8 + b] = vs;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                s_cnt[wave][b] = vp[b];
+                s_cnt[wave][8 + b] = vs[b];
+            }
         }
     }
     const uint32_t ninf = (uint32_t)__popc(inf_mask);
