@@ -432,19 +432,22 @@ __device__ __forceinline__ bool select_block(const Model& M, uint64_t eoff, cons
         total += s_w[w];
     }
     pos += inc - cnt;
-    __shared__ uint32_t s_k[kSelThreads * NK];
-    __shared__ int32_t s_i[kSelThreads * NK];
+    // Staged at j + j / 16: dense blocks put thread t's pairs at 16t + i,
+    // which the padding spreads over distinct LDS banks.
+    constexpr int SN = kSelThreads * NK + kSelThreads * NK / 16;
+    __shared__ uint32_t s_k[SN];
+    __shared__ int32_t s_i[SN];
 #pragma unroll
     for (int i = 0; i < NK; ++i)
         if (sel & (1u << i)) {
-            s_k[pos] = key[i];
-            s_i[pos] = (int32_t)(a0 + i / KPA);
+            s_k[pos + (pos >> 4)] = key[i];
+            s_i[pos + (pos >> 4)] = (int32_t)(a0 + i / KPA);
             pos++;
         }
     __syncthreads();
     for (uint32_t j = threadIdx.x; j < total; j += kSelThreads) {
-        sk[j] = s_k[j];
-        si[j] = s_i[j];
+        sk[j] = s_k[j + (j >> 4)];
+        si[j] = s_i[j + (j >> 4)];
     }
     if (cnt_out && threadIdx.x == 0) *cnt_out = total;
     return !bad;
