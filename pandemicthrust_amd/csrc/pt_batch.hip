@@ -376,7 +376,7 @@ __global__ __launch_bounds__(kSelThreads) void kb_emit(const DayState* __restric
                                                        const int32_t* __restrict__ stage_ids,
                                                        const uint32_t* __restrict__ blk_cnt,
                                                        const uint32_t* __restrict__ blk_off, uint32_t* __restrict__ sel_keys,
-                                                       int32_t* __restrict__ sel_ids) {
+                                                       int32_t* __restrict__ sel_ids, const uint32_t* __restrict__ seg_start) {
     if (g->done) return;
     constexpr int KPA = WEEKEND ? 3 : 2;
     constexpr int AG = kSelThreads * PER;
@@ -388,17 +388,24 @@ __global__ __launch_bounds__(kSelThreads) void kb_emit(const DayState* __restric
     const RepInfo ri = reps[r];
     const int64_t la = a - ri.abase;
     if (st->done || la >= ri.N) return;
-    const uint32_t off = blk_off[b];
+    // Global keys and dense positions, or (segmented tables) keys local to
+    // the replicate at its segment's positions.
+    const uint32_t kb = WEEKEND ? ri.kb_we : ri.kb_wd;
+    uint32_t off = blk_off[b], kbias = kb, ksub = 0;
+    if (seg_start) {
+        off = seg_start[r] + (off - blk_off[ri.abase / AG]);
+        kbias = 0;
+        ksub = kb;
+    }
     if (st->skip_full == 0) {
-        if (!select_block<PER, WEEKEND>(models[r], st->errand_off, age + ri.abase, mark,
-                                        WEEKEND ? ri.kb_we : ri.kb_wd, true, la, ri.N, sel_keys + off, sel_ids + off,
-                                        nullptr))
+        if (!select_block<PER, WEEKEND>(models[r], st->errand_off, age + ri.abase, mark, kbias, true, la, ri.N,
+                                        sel_keys + off, sel_ids + off, nullptr))
             st->fault = 1;
         return;
     }
     const uint32_t n = blk_cnt[b];
     for (uint32_t i = threadIdx.x; i < n; i += kSelThreads) {
-        sel_keys[off + i] = stage_keys[b * STAGE + i];
+        sel_keys[off + i] = stage_keys[b * STAGE + i] - ksub;
         sel_ids[off + i] = stage_ids[b * STAGE + i];
     }
 }
@@ -407,6 +414,43 @@ __global__ void kb_set_nsel(DayState* __restrict__ g, const uint32_t* __restrict
                             const uint32_t* __restrict__ blk_off, int nblk) {
     if (g->done) return;
     g->n_sel = blk_off[nblk - 1] + blk_cnt[nblk - 1];
+}
+
+// Segmented tables: each replicate's pair range for the day (the pairs of
+// its selection blocks, AG agents each, moved to a start rounded up to a
+// multiple of 4) and the day's pair count including those gaps.  One block
+// of kSegBlocks threads (R <= kSegBlocks).
+template <int AG>
+__global__ __launch_bounds__(kSegBlocks) void kb_segments(DayState* __restrict__ g, const RepInfo* __restrict__ reps,
+                                                          int R, int nb, const uint32_t* __restrict__ blk_cnt,
+                                                          const uint32_t* __restrict__ blk_off,
+                                                          uint32_t* __restrict__ seg_start,
+                                                          uint32_t* __restrict__ seg_count) {
+    if (g->done) return;
+    constexpr int NW = kSegBlocks / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ uint32_t s_w[NW];
+    uint32_t cnt = 0;
+    if (tid < R) {
+        const int64_t b0 = reps[tid].abase / AG;
+        const int64_t b1 = tid + 1 < R ? reps[tid + 1].abase / AG : (int64_t)nb;
+        if (b1 > b0) cnt = blk_off[b1 - 1] + blk_cnt[b1 - 1] - blk_off[b0];
+    }
+    const uint32_t al = (cnt + 3u) & ~3u;
+    const uint32_t inc = wave_incl_scan(al, lane);
+    if (lane == 63) s_w[wave] = inc;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        if (w < wave) base += s_w[w];
+        tot += s_w[w];
+    }
+    if (tid < R) {
+        seg_start[tid] = base + inc - al;
+        seg_count[tid] = cnt;
+    }
+    if (tid == 0) g->n_sel = tot;
 }
 
 template <bool WEEKEND>
@@ -499,6 +543,15 @@ struct pt_batch {
     int32_t *stage_ids = nullptr, *sel_ids = nullptr;
     uint32_t *sblk_cnt = nullptr, *sblk_off = nullptr;
     int32_t *er_people = nullptr, *er_off = nullptr;
+    // Segmented day tables (R <= kSegBlocks): each replicate's pairs sorted
+    // in place over its local keys.  Static: the radix blocks of each
+    // replicate and its bucket ranges; per day: its pair ranges.
+    bool seg_tables = false;
+    int32_t *seg_blk = nullptr, *seg_blk0 = nullptr, *seg_nblk = nullptr;
+    uint32_t *seg_kb_wd = nullptr, *seg_kb_we = nullptr;  // R + 1
+    int32_t *seg_K_wd = nullptr, *seg_K_we = nullptr;     // R
+    uint32_t *seg_start = nullptr, *seg_count = nullptr;  // R
+    int32_t K_local_wd = 0, K_local_we = 0;
     int32_t *d_counts = nullptr, *d_inf = nullptr;
     unsigned long long* d_scratch64 = nullptr;  // max(R, 512)
     TableScratch tables;
@@ -550,6 +603,16 @@ void free_batch_population(pt_batch* b) {
     dev_free(b->sblk_off);
     dev_free(b->er_people);
     dev_free(b->er_off);
+    dev_free(b->seg_blk);
+    dev_free(b->seg_blk0);
+    dev_free(b->seg_nblk);
+    dev_free(b->seg_kb_wd);
+    dev_free(b->seg_kb_we);
+    dev_free(b->seg_K_wd);
+    dev_free(b->seg_K_we);
+    dev_free(b->seg_start);
+    dev_free(b->seg_count);
+    b->seg_tables = false;
     dev_free(b->d_counts);
     dev_free(b->d_inf);
     table_scratch_free(b->tables);
@@ -632,18 +695,40 @@ int enqueue_batch_day(pt_batch* b, int d) {
                                                                 b->sblk_cnt);
     hipError_t e = exclusive_scan_u32(b->sblk_cnt, b->sblk_off, nb, b->tables.scan_part, &g->done, s);
     if (e != hipSuccess) return fail(PT_EDEVICE, "selection scan: %s", hipGetErrorString(e));
-    kb_set_nsel<<<1, 1, 0, s>>>(g, b->sblk_cnt, b->sblk_off, nb);
+    const uint32_t* seg_start = b->seg_tables ? b->seg_start : nullptr;
+    if (!b->seg_tables)
+        kb_set_nsel<<<1, 1, 0, s>>>(g, b->sblk_cnt, b->sblk_off, nb);
+    else if (weekend)
+        kb_segments<kSelAgentsWe><<<1, kSegBlocks, 0, s>>>(g, b->d_reps, R, nb, b->sblk_cnt, b->sblk_off, b->seg_start,
+                                                           b->seg_count);
+    else
+        kb_segments<kSelAgentsWd><<<1, kSegBlocks, 0, s>>>(g, b->d_reps, R, nb, b->sblk_cnt, b->sblk_off, b->seg_start,
+                                                           b->seg_count);
     if (weekend)
         kb_emit<kSelPerWe, true><<<nb, kSelThreads, 0, s>>>(g, b->d_states, b->d_reps, b->d_models, b->tile_rep, b->age,
                                                              b->mark, b->stage_keys, b->stage_ids, b->sblk_cnt, b->sblk_off,
-                                                             b->sel_keys, b->sel_ids);
+                                                             b->sel_keys, b->sel_ids, seg_start);
     else
         kb_emit<kSelPerWd, false><<<nb, kSelThreads, 0, s>>>(g, b->d_states, b->d_reps, b->d_models, b->tile_rep,
                                                               b->age, b->mark, b->stage_keys, b->stage_ids, b->sblk_cnt,
-                                                              b->sblk_off, b->sel_keys, b->sel_ids);
-    const int64_t n_cap = (weekend ? 3 : 2) * b->P;
-    e = build_table(b->tables, b->sel_keys, b->sel_ids, n_cap, &g->n_sel, (int32_t)K, ID_IDENTITY, 0, b->er_people,
-                    b->er_off, &g->done, s, nullptr);
+                                                              b->sblk_off, b->sel_keys, b->sel_ids, seg_start);
+    const int64_t n_cap = (weekend ? 3 : 2) * b->P + 4 * (int64_t)R;
+    if (b->seg_tables) {
+        SegLayout sl;
+        sl.blk_seg = b->seg_blk;
+        sl.seg_blk0 = b->seg_blk0;
+        sl.seg_nblk = b->seg_nblk;
+        sl.seg_start = b->seg_start;
+        sl.seg_count = b->seg_count;
+        sl.seg_kbase = weekend ? b->seg_kb_we : b->seg_kb_wd;
+        sl.seg_K = weekend ? b->seg_K_we : b->seg_K_wd;
+        sl.R = R;
+        e = build_table_segmented(b->tables, b->sel_keys, b->sel_ids, n_cap, weekend ? b->K_local_we : b->K_local_wd,
+                                  (int32_t)K, sl, b->er_people, b->er_off, &g->done, s, nullptr);
+    } else {
+        e = build_table(b->tables, b->sel_keys, b->sel_ids, n_cap, &g->n_sel, (int32_t)K, ID_IDENTITY, 0,
+                        b->er_people, b->er_off, &g->done, s, nullptr);
+    }
     if (e != hipSuccess) return fail(PT_EDEVICE, "batch table build: %s", hipGetErrorString(e));
     if (weekend)
         kb_contacts<true><<<b->grid_contacts, kContactThreads, 0, s>>>(g, b->d_states, b->d_reps, b->d_models,
@@ -794,8 +879,10 @@ int pt_batch_setup(pt_batch* b, int64_t* n_people_out) {
         Hmax = std::max<int64_t>(Hmax, m.H);
         ri.kb_wd = (uint32_t)kwd;
         ri.kb_we = (uint32_t)kwe;
-        kwd += ((uint64_t)m.wd_K + 31) / 32 * 32;
-        kwe += ((uint64_t)kWeHours * m.we_E + 31) / 32 * 32;
+        // At least one unqueried bucket after each replicate's: in segmented
+        // tables it takes the alignment gap after the replicate's pairs.
+        kwd += ((uint64_t)m.wd_K + 32) / 32 * 32;
+        kwe += ((uint64_t)kWeHours * m.we_E + 32) / 32 * 32;
         ri.workplace_off = (uint64_t)(m.H / 4);
         b->rand_after[r] = ri.workplace_off + (uint64_t)(N / 4);
     }
@@ -812,7 +899,7 @@ int pt_batch_setup(pt_batch* b, int64_t* n_people_out) {
     for (int r = 0; r < R; ++r)
         for (int t = 0; t < b->reps[r].ntiles; ++t) trep[b->reps[r].tile0 + t] = r;
     const int64_t kmax = std::max<int64_t>(kwd, kwe);
-    const int64_t n3 = 3 * P;
+    const int64_t n3 = 3 * P + 4 * (int64_t)R;  // segment starts are rounded up to multiples of 4
     const int64_t nsel = P / kSelAgentsWe + 1;
     if (dev_alloc(&b->tile_rep, tiles) || dev_alloc(&b->st_p, P) || dev_alloc(&b->st_s, P) ||
         dev_alloc(&b->dy_p, P) || dev_alloc(&b->dy_s, P) || dev_alloc(&b->gn_p, P) || dev_alloc(&b->gn_s, P) ||
@@ -825,7 +912,10 @@ int pt_batch_setup(pt_batch* b, int64_t* n_people_out) {
         dev_alloc(&b->stage_ids, n3) || dev_alloc(&b->sel_keys, n3) || dev_alloc(&b->sel_ids, n3) ||
         dev_alloc(&b->sblk_cnt, nsel) || dev_alloc(&b->sblk_off, nsel) || dev_alloc(&b->er_people, n3) ||
         dev_alloc(&b->er_off, kmax + 1) || dev_alloc(&b->d_counts, (int64_t)R * b->D * 16) ||
-        dev_alloc(&b->d_inf, (int64_t)R * b->D)) {
+        dev_alloc(&b->d_inf, (int64_t)R * b->D) || dev_alloc(&b->seg_blk, kSegBlocks) ||
+        dev_alloc(&b->seg_blk0, R) || dev_alloc(&b->seg_nblk, R) || dev_alloc(&b->seg_kb_wd, R + 1) ||
+        dev_alloc(&b->seg_kb_we, R + 1) || dev_alloc(&b->seg_K_wd, R) || dev_alloc(&b->seg_K_we, R) ||
+        dev_alloc(&b->seg_start, R) || dev_alloc(&b->seg_count, R)) {
         const std::string msg = pt_last_error();
         free_batch_population(b);
         return fail(PT_ENOMEM, "%s", msg.c_str());
@@ -839,6 +929,54 @@ int pt_batch_setup(pt_batch* b, int64_t* n_people_out) {
     HIP_TRY(hipMemcpyAsync(b->tile_rep, trep.data(), sizeof(int32_t) * tiles, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(b->d_models, b->models.data(), sizeof(Model) * R, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(b->d_reps, b->reps.data(), sizeof(RepInfo) * R, hipMemcpyHostToDevice, st));
+    // Segmented tables: radix blocks per replicate in proportion to its
+    // agents (at least one each), and each replicate's bucket range.
+    b->seg_tables = R <= kSegBlocks;
+    if (b->seg_tables) {
+        std::vector<int32_t> nbr(R), blk0(R), blk(kSegBlocks), Kwd(R), Kwe(R);
+        std::vector<uint32_t> kbwd(R + 1), kbwe(R + 1);
+        int used = 0;
+        for (int r = 0; r < R; ++r) {
+            nbr[r] = std::max<int32_t>(1, (int32_t)((double)kSegBlocks * b->reps[r].N / (double)n_agents));
+            used += nbr[r];
+        }
+        while (used > kSegBlocks) {  // take from the replicate with the most blocks
+            const int r = (int)(std::max_element(nbr.begin(), nbr.end()) - nbr.begin());
+            nbr[r]--;
+            used--;
+        }
+        while (used < kSegBlocks) {  // give to the replicate with the most agents per block
+            int best = 0;
+            for (int r = 1; r < R; ++r)
+                if ((double)b->reps[r].N * nbr[best] > (double)b->reps[best].N * nbr[r]) best = r;
+            nbr[best]++;
+            used++;
+        }
+        int acc = 0;
+        for (int r = 0; r < R; ++r) {
+            Kwd[r] = b->models[r].wd_K;
+            Kwe[r] = kWeHours * b->models[r].we_E;
+            kbwd[r] = b->reps[r].kb_wd;
+            kbwe[r] = b->reps[r].kb_we;
+        }
+        kbwd[R] = b->K_wd;
+        kbwe[R] = b->K_we;
+        b->K_local_wd = *std::max_element(Kwd.begin(), Kwd.end());
+        b->K_local_we = *std::max_element(Kwe.begin(), Kwe.end());
+        for (int r = 0; r < R; ++r) {
+            blk0[r] = acc;
+            for (int j = 0; j < nbr[r]; ++j) blk[acc + j] = r;
+            acc += nbr[r];
+        }
+        HIP_TRY(hipMemcpyAsync(b->seg_blk, blk.data(), sizeof(int32_t) * kSegBlocks, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(b->seg_blk0, blk0.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(b->seg_nblk, nbr.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(b->seg_kb_wd, kbwd.data(), sizeof(uint32_t) * (R + 1), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(b->seg_kb_we, kbwe.data(), sizeof(uint32_t) * (R + 1), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(b->seg_K_wd, Kwd.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(b->seg_K_we, Kwe.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipStreamSynchronize(st));  // the host vectors above are about to go
+    }
     // Padding slots keep status 0 and are never read; zero them for tidiness.
     HIP_TRY(hipMemsetAsync(b->st_p, 0, (size_t)P, st));
     HIP_TRY(hipMemsetAsync(b->st_s, 0, (size_t)P, st));

@@ -109,6 +109,36 @@ hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int
                        int32_t K, IdMode mode, int32_t N, int32_t* people_out, int32_t* offsets_out,
                        const int32_t* done, hipStream_t stream, LaunchTimer* tm, uint32_t drop_key = kNoDrop);
 
+// Segmented tables: the tables of R independent simulations built by one
+// sort whose passes cover only the bits of the largest local key.  Segment r
+// holds its pairs at [seg_start[r], seg_start[r] + seg_count[r]) with keys
+// local to it (< seg_K[r]) and is sorted in place; seg_start values are
+// multiples of 4 and the segments lie in order.  Radix block b works inside
+// segment blk_seg[b] only; segment r owns the blocks [seg_blk0[r],
+// seg_blk0[r] + seg_nblk[r]) of the kSegBlocks blocks of a pass.  The
+// offsets are global: offsets[seg_kbase[r] + q] for q in [0, seg_K[r]] are
+// positions of the shared people array; buckets between seg_kbase[r] +
+// seg_K[r] and seg_kbase[r + 1] hold no pairs.  Requires seg_kbase[r] +
+// seg_K[r] < seg_kbase[r + 1]: the last of those buckets spans the segment's
+// alignment gap (unwritten positions) and must not be queried.
+constexpr int kSegBlocks = 1024;
+struct SegLayout {
+    const int32_t* blk_seg = nullptr;     // kSegBlocks
+    const int32_t* seg_blk0 = nullptr;    // R
+    const int32_t* seg_nblk = nullptr;    // R
+    const uint32_t* seg_start = nullptr;  // R (device, per build)
+    const uint32_t* seg_count = nullptr;  // R (device, per build)
+    const uint32_t* seg_kbase = nullptr;  // R + 1, seg_kbase[R] = the total bucket count
+    const int32_t* seg_K = nullptr;       // R
+    int R = 0;
+};
+
+// Builds every segment's table (ids from vals, explicit).  K_local: the
+// largest seg_K; K_total = seg_kbase[R].  n_cap bounds the positions used.
+hipError_t build_table_segmented(TableScratch& s, uint32_t* keys, const int32_t* vals, int64_t n_cap,
+                                 int32_t K_local, int32_t K_total, const SegLayout& seg, int32_t* people_out,
+                                 int32_t* offsets_out, const int32_t* done, hipStream_t stream, LaunchTimer* tm);
+
 // Number of radix passes build_table uses for K buckets at max_bits per pass.
 int radix_passes(int32_t K, int max_bits);
 

@@ -58,6 +58,29 @@ __device__ __forceinline__ void block_range(int64_t n_host, const uint32_t* n_de
     end = min(n, begin + tpb * tile);
 }
 
+// Segmented passes: the block's slice of its segment (whole tiles, starting
+// at a multiple of 4), the segment's alignment gap (counted by its last block
+// in the top digit, so the scan leaves it after the segment's pairs), and its
+// histogram column (hist[hbase + d * hstride]: segment-major, so one
+// exclusive scan gives every segment its in-place offsets).
+template <int T, int RADIX>
+__device__ __forceinline__ void seg_range(const SegLayout& g, int64_t& begin, int64_t& end, uint32_t& gap,
+                                          int64_t& hbase, int& hstride) {
+    constexpr int64_t tile = (int64_t)T * kItems;
+    const int r = g.blk_seg[blockIdx.x];
+    const int j = (int)blockIdx.x - g.seg_blk0[r];
+    const int nb = g.seg_nblk[r];
+    const int64_t S = g.seg_start[r], cnt = g.seg_count[r];
+    const int64_t tiles = (cnt + tile - 1) / tile;
+    const int64_t tpb = (tiles + nb - 1) / nb;
+    begin = S + (int64_t)j * tpb * tile;
+    end = min(S + cnt, begin + tpb * tile);
+    if (begin >= end) begin = end = 0;  // no pairs (keeps the vector loads of the upsweep aligned)
+    gap = j == nb - 1 ? (uint32_t)(((cnt + 3) & ~(int64_t)3) - cnt) : 0u;
+    hbase = (int64_t)RADIX * g.seg_blk0[r] + j;
+    hstride = nb;
+}
+
 // Inclusive scan across a wave64.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
 #pragma unroll
@@ -98,7 +121,7 @@ template <int BITS, bool MATCH, bool PIN, int T>
 __global__ __launch_bounds__(T) void k_upsweep(const uint32_t* __restrict__ keys, const uint2* __restrict__ pairs,
                                                int64_t n_host, const uint32_t* __restrict__ n_dev, int shift,
                                                uint32_t* __restrict__ hist, int nblk, const int32_t* done,
-                                               uint32_t drop) {
+                                               uint32_t drop, SegLayout seg) {
     if (is_done(done)) return;
     constexpr int RADIX = 1 << BITS;
     constexpr int W = T / 64;
@@ -108,8 +131,13 @@ __global__ __launch_bounds__(T) void k_upsweep(const uint32_t* __restrict__ keys
     const int tid = threadIdx.x, wave = tid >> 6;
     for (int i = tid; i < W * RADIX; i += T) (&s_h[0][0])[i] = 0;
     __syncthreads();
-    int64_t begin, end;
-    block_range<T>(n_host, n_dev, begin, end);
+    int64_t begin, end, hbase = blockIdx.x;
+    int hstride = nblk;
+    uint32_t gap = 0;
+    if (seg.blk_seg)
+        seg_range<T, RADIX>(seg, begin, end, gap, hbase, hstride);
+    else
+        block_range<T>(n_host, n_dev, begin, end);
     const uint4* v4 = PIN ? reinterpret_cast<const uint4*>(pairs) : reinterpret_cast<const uint4*>(keys);
     // U 16-byte loads in flight per thread and iteration.
     constexpr int U = 4;
@@ -165,10 +193,10 @@ __global__ __launch_bounds__(T) void k_upsweep(const uint32_t* __restrict__ keys
     }
     __syncthreads();
     for (int d = tid; d < RADIX; d += T) {
-        uint32_t t = 0;
+        uint32_t t = d == RADIX - 1 ? gap : 0u;
 #pragma unroll
         for (int w = 0; w < W; ++w) t += s_h[w][d];
-        hist[(int64_t)d * nblk + blockIdx.x] = t;
+        hist[hbase + (int64_t)d * hstride] = t;
     }
 }
 
@@ -188,7 +216,8 @@ __global__ __launch_bounds__(T) void k_downsweep(const uint32_t* __restrict__ ke
                                                  int32_t* __restrict__ vals_out, uint2* __restrict__ pairs_out,
                                                  const uint32_t* __restrict__ hist, int64_t n_host,
                                                  const uint32_t* __restrict__ n_dev, int shift, int nblk,
-                                                 const int32_t* done, uint32_t drop, uint32_t* __restrict__ n_out) {
+                                                 const int32_t* done, uint32_t drop, uint32_t* __restrict__ n_out,
+                                                 SegLayout seg) {
     if (is_done(done)) return;
     constexpr int RADIX = 1 << BITS;
     constexpr int W = T / 64;
@@ -208,8 +237,15 @@ __global__ __launch_bounds__(T) void k_downsweep(const uint32_t* __restrict__ ke
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int64_t begin, end;
-    block_range<T>(n_host, n_dev, begin, end);
-    for (int d = tid; d < RADIX; d += T) s_base[d] = hist[(int64_t)d * nblk + blockIdx.x];
+    int64_t hbase = blockIdx.x;
+    int hstride = nblk;
+    if (seg.blk_seg) {
+        uint32_t gap;
+        seg_range<T, RADIX>(seg, begin, end, gap, hbase, hstride);
+    } else {
+        block_range<T>(n_host, n_dev, begin, end);
+    }
+    for (int d = tid; d < RADIX; d += T) s_base[d] = hist[hbase + (int64_t)d * hstride];
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
     auto load = [&](int64_t j, bool valid, uint32_t& k, int32_t& v) {
@@ -447,6 +483,42 @@ __global__ void k_offsets(const uint32_t* __restrict__ keys, int64_t n_host, con
     }
 }
 
+// Segmented offsets: bucket q of the global numbering belongs to the last
+// segment r with seg_kbase[r] <= q; offsets[q] = the first position of
+// segment r whose local key is >= q - seg_kbase[r] (its whole range when q
+// lies past the segment's buckets).
+__global__ void k_offsets_seg(const uint32_t* __restrict__ keys, SegLayout seg, int32_t K_total,
+                              int32_t* __restrict__ offsets, const int32_t* done) {
+    if (is_done(done)) return;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q <= K_total; q += stride) {
+        int lo = 0, hi = seg.R;  // last r in [0, R) with seg_kbase[r] <= q
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if ((int64_t)seg.seg_kbase[mid] <= q)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const int r = lo;
+        const int64_t S = seg.seg_start[r], cnt = seg.seg_count[r];
+        const int64_t lq = q - (int64_t)seg.seg_kbase[r];
+        int64_t a = S + cnt;  // at or past the segment's last bucket: its end
+        if (lq < seg.seg_K[r]) {
+            int64_t b = S + cnt;
+            a = S;
+            while (a < b) {
+                const int64_t mid = (a + b) >> 1;
+                if ((int64_t)keys[mid] < lq)
+                    a = mid + 1;
+                else
+                    b = mid;
+            }
+        }
+        offsets[q] = (int32_t)a;
+    }
+}
+
 // One radix pass.  Input: keys + vals (or implied ids) when pin is null,
 // else pairs pin; output: pairs pout when set, else keys kout + ids vout.
 struct PassIO {
@@ -461,29 +533,30 @@ struct PassIO {
 template <int BITS, int T>
 hipError_t radix_pass(TableScratch& s, const PassIO& io, int id_mode, int32_t N, int64_t n, const uint32_t* n_dev,
                       int shift, const int32_t* done, hipStream_t st, LaunchTimer* tm, uint32_t drop,
-                      uint32_t* n_out) {
+                      uint32_t* n_out, const SegLayout& seg) {
     const int64_t tiles = (n + (int64_t)T * kItems - 1) / ((int64_t)T * kItems);
-    const int nblk = (int)std::min<int64_t>(std::max<int64_t>(tiles, 1), max_blocks<T>());
+    const int nblk = seg.blk_seg ? kSegBlocks : (int)std::min<int64_t>(std::max<int64_t>(tiles, 1), max_blocks<T>());
     const bool pin = io.pin != nullptr;
-    if (pin && s.grouped_keys)
+    const bool grouped = s.grouped_keys && !seg.blk_seg;  // segment-local keys do not come in long runs
+    if (pin && grouped)
         launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, true, true, T>, dim3(nblk), dim3(T), st, io.kin, io.pin, n, n_dev,
-               shift, s.hist, nblk, done, drop);
+               shift, s.hist, nblk, done, drop, seg);
     else if (pin)
         launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, false, true, T>, dim3(nblk), dim3(T), st, io.kin, io.pin, n, n_dev,
-               shift, s.hist, nblk, done, drop);
-    else if (s.grouped_keys)
+               shift, s.hist, nblk, done, drop, seg);
+    else if (grouped)
         launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, true, false, T>, dim3(nblk), dim3(T), st, io.kin, io.pin, n, n_dev,
-               shift, s.hist, nblk, done, drop);
+               shift, s.hist, nblk, done, drop, seg);
     else
         launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, false, false, T>, dim3(nblk), dim3(T), st, io.kin, io.pin, n,
-               n_dev, shift, s.hist, nblk, done, drop);
+               n_dev, shift, s.hist, nblk, done, drop, seg);
     hipError_t e =
         exclusive_scan_u32(s.hist, s.hist, ((int64_t)nblk << BITS) + 1, s.scan_part, done, st, tm, KC_SORT_SCAN);
     if (e != hipSuccess) return e;
     const uint32_t* h = s.hist;
 #define PT_DOWN(PI, PO)                                                                                              \
     launch(tm, KC_SORT_DOWNSWEEP, k_downsweep<BITS, PI, PO, T>, dim3(nblk), dim3(T), st, io.kin, io.vin, io.pin,      \
-           id_mode, N, io.kout, io.vout, io.pout, h, n, n_dev, shift, nblk, done, drop, n_out)
+           id_mode, N, io.kout, io.vout, io.pout, h, n, n_dev, shift, nblk, done, drop, n_out, seg)
     if (pin && io.pout)
         PT_DOWN(true, true);
     else if (pin)
@@ -499,14 +572,15 @@ hipError_t radix_pass(TableScratch& s, const PassIO& io, int id_mode, int32_t N,
 // 1024-thread blocks exist for digits of up to 8 bits (LDS).
 hipError_t radix_pass_bits(int bits, TableScratch& s, const PassIO& io, int id_mode, int32_t N, int64_t n,
                            const uint32_t* n_dev, int shift, const int32_t* done, hipStream_t st, LaunchTimer* tm,
-                           uint32_t drop, uint32_t* n_out) {
-    const bool wide = bits <= 8 && bits >= s.wide_bits;
-#define PT_PASS2(B)                                                                                          \
-    case B:                                                                                                  \
-        return wide ? radix_pass<B, 1024>(s, io, id_mode, N, n, n_dev, shift, done, st, tm, drop, n_out)     \
-                    : radix_pass<B, 512>(s, io, id_mode, N, n, n_dev, shift, done, st, tm, drop, n_out);
+                           uint32_t drop, uint32_t* n_out, const SegLayout& seg) {
+    // Segmented passes always run kSegBlocks blocks of 512 threads.
+    const bool wide = !seg.blk_seg && bits <= 8 && bits >= s.wide_bits;
+#define PT_PASS2(B)                                                                                            \
+    case B:                                                                                                    \
+        return wide ? radix_pass<B, 1024>(s, io, id_mode, N, n, n_dev, shift, done, st, tm, drop, n_out, seg)  \
+                    : radix_pass<B, 512>(s, io, id_mode, N, n, n_dev, shift, done, st, tm, drop, n_out, seg);
 #define PT_PASS1(B) \
-    case B: return radix_pass<B, 512>(s, io, id_mode, N, n, n_dev, shift, done, st, tm, drop, n_out);
+    case B: return radix_pass<B, 512>(s, io, id_mode, N, n, n_dev, shift, done, st, tm, drop, n_out, seg);
     switch (bits) {
         PT_PASS2(1) PT_PASS2(2) PT_PASS2(3) PT_PASS2(4) PT_PASS2(5) PT_PASS2(6)
         PT_PASS2(7) PT_PASS2(8) PT_PASS1(9) PT_PASS1(10) PT_PASS1(11)
@@ -605,7 +679,7 @@ hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int
             }
             const bool dropping = p == 0 && drop_key != kNoDrop;
             hipError_t e = radix_pass_bits(pass_bits[p], s, io, mode, N, n, n_dev, shift, done, stream, tm,
-                                           dropping ? drop_key : kNoDrop, dropping ? s.n_kept : nullptr);
+                                           dropping ? drop_key : kNoDrop, dropping ? s.n_kept : nullptr, SegLayout{});
             if (e != hipSuccess) return e;
             // After a first pass that dropped keys the remaining passes
             // sort the pairs it kept (count on the device).
@@ -616,6 +690,42 @@ hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int
     // keys_b now holds the sorted keys.
     const int oblocks = (int)std::min<int64_t>(((int64_t)K + 512) / 512, 4096);
     launch(tm, KC_OFFSETS, k_offsets, dim3(oblocks), dim3(512), stream, (const uint32_t*)s.keys_b, n, n_dev, K,
+           offsets_out, done);
+    return hipGetLastError();
+}
+
+hipError_t build_table_segmented(TableScratch& s, uint32_t* keys, const int32_t* vals, int64_t n_cap,
+                                 int32_t K_local, int32_t K_total, const SegLayout& seg, int32_t* people_out,
+                                 int32_t* offsets_out, const int32_t* done, hipStream_t stream, LaunchTimer* tm) {
+    if (n_cap > s.n_cap || K_local < 1 || K_total < 1 || !vals || !seg.blk_seg || seg.R < 1 || seg.R > kSegBlocks)
+        return hipErrorInvalidValue;
+    const int bits = key_bits(K_local);
+    const int passes = radix_passes(K_local, s.max_bits);
+    int pass_bits[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int p = 0; p < passes; ++p) pass_bits[p] = bits / passes + (p < bits % passes ? 1 : 0);
+    int shift = 0;
+    for (int p = 0; p < passes; ++p) {
+        const bool last = p == passes - 1;
+        PassIO io{};
+        if (p == 0) {
+            io.kin = keys;
+            io.vin = vals;
+        } else {
+            io.pin = (p & 1) ? s.pairs_a : s.pairs_b;
+        }
+        if (last) {
+            io.kout = s.keys_b;
+            io.vout = people_out;
+        } else {
+            io.pout = (p & 1) ? s.pairs_b : s.pairs_a;
+        }
+        hipError_t e = radix_pass_bits(pass_bits[p], s, io, ID_IDENTITY, 0, n_cap, nullptr, shift, done, stream, tm,
+                                       kNoDrop, nullptr, seg);
+        if (e != hipSuccess) return e;
+        shift += pass_bits[p];
+    }
+    const int oblocks = (int)std::min<int64_t>(((int64_t)K_total + 512) / 512, 4096);
+    launch(tm, KC_OFFSETS, k_offsets_seg, dim3(oblocks), dim3(512), stream, (const uint32_t*)s.keys_b, seg, K_total,
            offsets_out, done);
     return hipGetLastError();
 }

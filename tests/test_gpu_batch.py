@@ -145,3 +145,23 @@ def test_batch_of_one_equals_single_simulation():
     _assert_rep_equals_solo(b, 0, res[0], s, s.run(), "single")
     s.close()
     b.close()
+
+
+def test_replicates_whose_weekend_pairs_leave_alignment_gaps():
+    """Each replicate's pairs start at a multiple of 4 in the shared table.
+    These populations (29729, 29919, 29925 agents) have 3N weekend pairs not
+    divisible by 4, so unwritten positions follow the first two replicates,
+    and their weekend bucket count (160) is a multiple of 32.  Regression for
+    the last real bucket running into those positions: every replicate must
+    equal its single run over 40 days."""
+    row = ["40", "1.8", "1.3", "10", "10", "0.0127", "0.0127", "1", "0"]
+    rows = [(row, 9), (row, 10), (row, 11)]
+    b, n = _batch(rows, 1)
+    assert [(3 * int(x)) % 4 for x in n] == [3, 1, 3]
+    res = b.run()
+    for r, (rw, seed) in enumerate(rows):
+        s = _solo(rw, seed)
+        s.set_table_mode(1)
+        _assert_rep_equals_solo(b, r, res[r], s, s.run(), "gaps")
+        s.close()
+    b.close()
