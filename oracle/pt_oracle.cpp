@@ -8,7 +8,16 @@
  * -ffp-contract=off and SSE arithmetic so no operation is fused or widened.
  * Float->int conversions use GPU semantics (truncate, saturate, NaN->0).
  *
- * Pinning status of each part is recorded in DESIGN.md of the repository.
+ * How each part is pinned (tests/test_oracle_golden.py and the parity tests):
+ *   Threefry             published Random123 KATs + vectors of the vendored header
+ *   model tables         every literal of the reference's setup_loadParameters()
+ *   scaling, workplace, weekday/weekend errands, fishing walks
+ *                        vectors of the reference's own host_functions.cpp
+ *   household-type walk  numpy inverse CDF over the reference's literals (ties,
+ *                        neighbours of every boundary, random draws)
+ *   tables, counts, R    closed forms (stable bucket order, lower_bound, totals)
+ *   contacts, infection  R = 0 closed form, profile-range special cases, the
+ *                        reference's output file contract
  */
 #include "pt_oracle.h"
 

@@ -245,3 +245,7 @@ def raw_table(name):
     if n < 0:
         raise KeyError(name)
     return out[:n].copy()
+
+
+def fish_household_type(r):
+    return int(lib().or_fish_household_type(C.c_uint32(r)))

@@ -155,3 +155,35 @@ def test_fish_functions_golden(cfg):
         assert sim.fish_weekend_dest(u) == int(r["weekend"]), r
         n += 1
     assert n >= 700
+
+
+def test_household_type_walk_is_the_inverse_cdf():
+    """device_setup_fishHouseholdType (PandemicSim.cu:2360-2371) has no host
+    twin in the reference, so the oracle's walk is pinned by what the
+    mathematics fixes: for the reference's float32 CDF literals
+    (ref_tables.csv, hh_cdf) and y = float(r) / float(0xFFFFFFFF), the walk
+    returns the first row whose CDF value is >= y, clipped to the last row
+    (numpy's searchsorted, side='left').  The draws include every r whose y
+    equals a CDF value exactly (a '>=' for '>' fails there), its float32
+    neighbours on both sides (an off-by-one row fails), the extremes and
+    random draws."""
+    cdf = np.array([float(r["value"]) for r in rows("ref_tables.csv") if r["name"] == "hh_cdf"], np.float32)
+    assert len(cdf) == 9 and cdf[-1] == np.float32(1.0)
+    scale = np.float32(np.uint32(0xFFFFFFFF))  # rounds to 2**32
+    assert float(scale) == 2.0 ** 32
+    rs = [0, 1, 2**31, 2**32 - 1, 2**32 - 128, 2**32 - 129]
+    for c in cdf[:-1]:
+        r0 = int(np.float64(c) * 2.0**32)  # exact: c * 2**32 is an integer (float32 c < 1)
+        assert np.float32(r0) / scale == c
+        rs += [r0 + d for d in range(-300, 301, 1)]
+    rs += [int(x) for x in np.random.default_rng(7).integers(0, 2**32, 20000, dtype=np.uint64)]
+    rs = np.array(sorted(set(r for r in rs if 0 <= r < 2**32)), np.uint64)
+    y = rs.astype(np.float32) / scale
+    expect = np.minimum(np.searchsorted(cdf, y, side="left"), len(cdf) - 1)
+    got = np.array([O.fish_household_type(int(r)) for r in rs])
+    bad = np.nonzero(got != expect)[0]
+    assert len(bad) == 0, [(int(rs[i]), float(y[i]), int(got[i]), int(expect[i])) for i in bad[:5]]
+    # every row is reachable and the exact ties land on the row they close
+    assert set(got.tolist()) == set(range(len(cdf)))
+    for i, c in enumerate(cdf[:-1]):
+        assert O.fish_household_type(int(np.float64(c) * 2.0**32)) == i
