@@ -253,6 +253,10 @@ enum pt_launch_knob {
                                             K * value / 100 (0..100, default 75) */
     PT_LAUNCH_SORT_MAX_BITS,             /* widest digit of a radix pass: K buckets take ceil(log2 K / value)
                                             passes (4..11; default 8, 9 in a replicate batch) */
+    PT_LAUNCH_SEGMENTED_TABLES,          /* replicate batch: 1 = each replicate's day table sorted in its own
+                                            segment over its local keys (batches of up to 1024 replicates),
+                                            0 = one sort over every replicate's key range (0..1, default 1;
+                                            a single simulation ignores it) */
     PT_LAUNCH_KNOBS
 };
 int pt_sim_set_launch(pt_sim* sim, int32_t knob, int32_t value);

@@ -40,7 +40,7 @@ EXPORTS = [
 
 # enum pt_launch_knob (include/pandemic/pt_abi.h)
 LAUNCH_KNOBS = ["contact_blocks_per_cu", "gen_blocks_per_cu", "sort_wide_bits", "mark_limit_pct", "subset_share_pct",
-                "sort_max_bits"]
+                "sort_max_bits", "segmented_tables"]
 
 # enum pt_setup_phase (include/pandemic/pt_abi.h)
 SETUP_PHASES = ["scale", "push", "population_size", "size_arrays", "households", "workplaces", "status",

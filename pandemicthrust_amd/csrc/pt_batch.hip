@@ -557,7 +557,7 @@ struct pt_batch {
     TableScratch tables;
     int grid_contacts = 2048;
     int cus = 256;                                   // compute units of the device
-    int launch[PT_LAUNCH_KNOBS] = {8, 16, 8, 100, 75, 9};  // pt_launch_knob values
+    int launch[PT_LAUNCH_KNOBS] = {8, 16, 8, 100, 75, 9, 1};  // pt_launch_knob values
     hipEvent_t run_begin = nullptr, run_end = nullptr;
     // The day step captured once per layout as HIP graphs (weekday,
     // weekend) and replayed for every day; the kernels take the day and
@@ -695,8 +695,9 @@ int enqueue_batch_day(pt_batch* b, int d) {
                                                                 b->sblk_cnt);
     hipError_t e = exclusive_scan_u32(b->sblk_cnt, b->sblk_off, nb, b->tables.scan_part, &g->done, s);
     if (e != hipSuccess) return fail(PT_EDEVICE, "selection scan: %s", hipGetErrorString(e));
-    const uint32_t* seg_start = b->seg_tables ? b->seg_start : nullptr;
-    if (!b->seg_tables)
+    const bool seg = b->seg_tables && b->launch[PT_LAUNCH_SEGMENTED_TABLES] != 0;
+    const uint32_t* seg_start = seg ? b->seg_start : nullptr;
+    if (!seg)
         kb_set_nsel<<<1, 1, 0, s>>>(g, b->sblk_cnt, b->sblk_off, nb);
     else if (weekend)
         kb_segments<kSelAgentsWe><<<1, kSegBlocks, 0, s>>>(g, b->d_reps, R, nb, b->sblk_cnt, b->sblk_off, b->seg_start,
@@ -713,7 +714,7 @@ int enqueue_batch_day(pt_batch* b, int d) {
                                                               b->age, b->mark, b->stage_keys, b->stage_ids, b->sblk_cnt,
                                                               b->sblk_off, b->sel_keys, b->sel_ids, seg_start);
     const int64_t n_cap = (weekend ? 3 : 2) * b->P + 4 * (int64_t)R;
-    if (b->seg_tables) {
+    if (seg) {
         SegLayout sl;
         sl.blk_seg = b->seg_blk;
         sl.seg_blk0 = b->seg_blk0;
@@ -929,8 +930,9 @@ int pt_batch_setup(pt_batch* b, int64_t* n_people_out) {
     HIP_TRY(hipMemcpyAsync(b->tile_rep, trep.data(), sizeof(int32_t) * tiles, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(b->d_models, b->models.data(), sizeof(Model) * R, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(b->d_reps, b->reps.data(), sizeof(RepInfo) * R, hipMemcpyHostToDevice, st));
-    // Segmented tables: radix blocks per replicate in proportion to its
-    // agents (at least one each), and each replicate's bucket range.
+    // Segmented tables (used unless PT_LAUNCH_SEGMENTED_TABLES is 0): radix
+    // blocks per replicate in proportion to its agents (at least one each),
+    // and each replicate's bucket range.
     b->seg_tables = R <= kSegBlocks;
     if (b->seg_tables) {
         std::vector<int32_t> nbr(R), blk0(R), blk(kSegBlocks), Kwd(R), Kwe(R);

@@ -56,7 +56,7 @@ int fail(int code, const char* fmt, ...) {
 
 int check_launch(int32_t knob, int32_t value) {
     if (knob < 0 || knob >= PT_LAUNCH_KNOBS) return fail(PT_EINVAL, "launch setting %d unknown", knob);
-    const int lo[PT_LAUNCH_KNOBS] = {1, 1, 1, 0, 0, 4}, hi[PT_LAUNCH_KNOBS] = {64, 64, 9, 1000, 100, 11};
+    const int lo[PT_LAUNCH_KNOBS] = {1, 1, 1, 0, 0, 4, 0}, hi[PT_LAUNCH_KNOBS] = {64, 64, 9, 1000, 100, 11, 1};
     if (value < lo[knob] || value > hi[knob])
         return fail(PT_EINVAL, "launch setting %d: value %d outside [%d, %d]", knob, value, lo[knob], hi[knob]);
     return PT_OK;
@@ -834,7 +834,7 @@ struct pt_sim {
     int32_t *tr_victim = nullptr, *tr_loc = nullptr;
     uint8_t *tr_type = nullptr, *tr_attempt = nullptr, *tr_action = nullptr;
     int cus = 128;                                   // compute units of the device
-    int launch[PT_LAUNCH_KNOBS] = {8, 16, 8, 100, 75, 8};  // pt_launch_knob values
+    int launch[PT_LAUNCH_KNOBS] = {8, 16, 8, 100, 75, 8, 1};  // pt_launch_knob values
 };
 
 namespace {

@@ -165,3 +165,19 @@ def test_replicates_whose_weekend_pairs_leave_alignment_gaps():
         _assert_rep_equals_solo(b, r, res[r], s, s.run(), "gaps")
         s.close()
     b.close()
+
+
+def test_global_tables_match_single_simulations():
+    """Batches of more than 1024 replicates build one table over every
+    replicate's key range (PT_LAUNCH_SEGMENTED_TABLES = 0 forces that path
+    here): every replicate must still equal its single run."""
+    from pandemicthrust_amd import PandemicBatch, Params
+    b = PandemicBatch([Params.from_row(r) for r, _ in ROWS], [s for _, s in ROWS])
+    b.set_launch("segmented_tables", 0)
+    b.setup()
+    res = b.run()
+    for r, (row, seed) in enumerate(ROWS):
+        s = _solo(row, seed)
+        _assert_rep_equals_solo(b, r, res[r], s, s.run(), "global tables")
+        s.close()
+    b.close()

@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 
 ROW = ["40", "1.8", "1.3", "10", "10", "0.0425", "0.0425", "1", "0"]
 DEFAULTS = {"contact_blocks_per_cu": 8, "gen_blocks_per_cu": 16, "sort_wide_bits": 8, "mark_limit_pct": 100,
-            "subset_share_pct": 75, "sort_max_bits": 8}
+            "subset_share_pct": 75, "sort_max_bits": 8, "segmented_tables": 1}
 ROWS = [(["30", "2.0", "1.4", "10", "10", "0.03", "0.03", "1", "0"], 1),
         (["35", "1.7", "1.3", "20", "5", "0.05", "0.04", "0.9", "0.2"], 2),
         (["25", "2.4", "1.2", "8", "12", "0.02", "0.025", "1", "0"], 3)]
@@ -51,7 +51,7 @@ def test_batch_launch_settings_do_not_change_results():
     ref = b.run()
     for settings in ({"contact_blocks_per_cu": 1, "sort_wide_bits": 9}, {"contact_blocks_per_cu": 32, "sort_wide_bits": 2},
                      {"mark_limit_pct": 1000, "subset_share_pct": 100}, {"mark_limit_pct": 0, "subset_share_pct": 0},
-                     {"sort_max_bits": 4}, {"sort_max_bits": 11}):
+                     {"sort_max_bits": 4}, {"sort_max_bits": 11}, {"segmented_tables": 0}, {"sort_max_bits": 9}):
         for k, v in settings.items():
             b.set_launch(k, v)
         b.restore()
@@ -74,6 +74,8 @@ def test_launch_settings_are_checked():
         s.set_launch("subset_share_pct", 101)
     with pytest.raises(PTError, match="outside"):
         s.set_launch("sort_max_bits", 12)
+    with pytest.raises(PTError, match="outside"):
+        s.set_launch("segmented_tables", 2)
     s.close()
 
 
@@ -82,8 +84,10 @@ def test_autotuner_keeps_results():
     s.setup_sim()
     ref_c, ref_i = None, None
     best, log = autotune.tune_sim(s)
-    assert set(best) == set(DEFAULTS) == set(autotune.SIM_KNOBS)
-    assert len(log) == sum(len(autotune.CANDIDATES[k]) for k in best)
+    # every setting comes back; all but segmented_tables (batch only) are tuned
+    assert set(best) == set(DEFAULTS)
+    assert set(autotune.SIM_KNOBS) == set(DEFAULTS) - {"segmented_tables"}
+    assert len(log) == sum(len(autotune.CANDIDATES[k]) for k in autotune.SIM_KNOBS)
     for k, v in best.items():         # the default unless a candidate was clearly faster
         ms = {val: t for name, val, t in log if name == k}
         assert v == DEFAULTS[k] or \
