@@ -181,3 +181,23 @@ def test_global_tables_match_single_simulations():
         _assert_rep_equals_solo(b, r, res[r], s, s.run(), "global tables")
         s.close()
     b.close()
+
+
+@pytest.mark.parametrize("R", [1024, 1025])
+def test_largest_segmented_batch_and_the_first_global_one(R):
+    """1024 replicates: each owns exactly one radix block of the segmented
+    sort (its pairs and its alignment gap in one block); 1025: the batch
+    falls back to one global sort.  Sampled replicates must equal their
+    single runs."""
+    from pandemicthrust_amd import PandemicBatch, Params
+    row = ["6", "1.8", "1.3", "10", "10", "0.002", "0.002", "1", "0"]
+    seeds = [1000 + r for r in range(R)]
+    b = PandemicBatch([Params.from_row(row)] * R, seeds)
+    n = b.setup()
+    assert len(n) == R and min(n) > 1000
+    res = b.run()
+    for r in (0, 1, R // 2, R - 2, R - 1):
+        s = _solo(row, seeds[r])
+        _assert_rep_equals_solo(b, r, res[r], s, s.run(), "R=%d" % R)
+        s.close()
+    b.close()
