@@ -420,6 +420,8 @@ __global__ void kb_set_nsel(DayState* __restrict__ g, const uint32_t* __restrict
 // its selection blocks, AG agents each, moved to a start rounded up to a
 // multiple of 4) and the day's pair count including those gaps.  One block
 // of kSegBlocks threads (R <= kSegBlocks).
+static_assert(kSweepTile % kSelAgentsWd == 0 && kSweepTile % kSelAgentsWe == 0,
+              "replicates (whole sweep tiles) start on selection-block boundaries");
 template <int AG>
 __global__ __launch_bounds__(kSegBlocks) void kb_segments(DayState* __restrict__ g, const RepInfo* __restrict__ reps,
                                                           int R, int nb, const uint32_t* __restrict__ blk_cnt,
