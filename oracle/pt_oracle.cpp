@@ -16,9 +16,29 @@
  *   household-type walk  numpy inverse CDF over the reference's literals (ties,
  *                        neighbours of every boundary, random draws)
  *   tables, counts, R    closed forms (stable bucket order, lower_bound, totals)
- *   contacts, infection  R = 0 closed form, profile-range special cases, the
+ *   contacts, infection  per-contact binomial rates against shed * factor /
+ *                        max(kval_sum, 1) and the R calibration, profile
+ *                        special cases (pct = 1, pct = 0, y == pct), the
+ *                        two-success profile property of the xor word, the
+ *                        min-(position, slot) winner rule, R = 0, the
  *                        reference's output file contract
+ *                        (tests/test_oracle_infection_pins.py)
+ *
+ * OR_PLANT (compile-time, tests only): 0 = the restatement.  Values 1..8
+ * build a copy with one deliberate, plausible mistake in the infection
+ * arithmetic so that the tests can show each pin fails on it:
+ *   1 profile threshold y >= pct instead of y > pct
+ *   2 shedding value read one day late
+ *   3 profile word u[i] ^ u[(i + 2) % 16] instead of u[i] ^ u[(i + 1) % 16]
+ *   4 contact weights normalised by kval_sum + 1
+ *   5 strain factor R * (1 - a) * pct instead of R / ((1 - a) * pct)
+ *   6 victim generation = infector generation (no + 1)
+ *   7 an infector's slots processed last to first
+ *   8 profile word u[i] (no xor)
  */
+#ifndef OR_PLANT
+#define OR_PLANT 0
+#endif
 #include "pt_oracle.h"
 
 #include <cmath>
@@ -189,7 +209,7 @@ struct or_sim {
     /* trace */
     bool trace_on = false;
     std::vector<int32_t> tr_infector, tr_victim;
-    std::vector<uint8_t> tr_type, tr_action;
+    std::vector<uint8_t> tr_type, tr_action, tr_draw, tr_prof;
 };
 
 namespace {
@@ -220,7 +240,11 @@ void scale_and_calibrate(or_sim* s) {
     for (int i = 0; i < 2; ++i) {
         volatile float a = 1.0f - s->p.asymp_factor;
         volatile float d = a * s->p.percent_symptomatic;
+#if OR_PLANT == 5
+        s->factor[i] = fmul(r[i], d);
+#else
         s->factor[i] = fdiv(r[i], d);
+#endif
     }
     s->pct = s->p.percent_symptomatic;
 }
@@ -477,7 +501,15 @@ int make_contacts_weekend(const or_sim* s, int32_t idx, uint64_t M, int32_t* vic
 /* device_calculateInfectionProbability (PandemicSim.cu:2307-2326) */
 float infection_probability(const or_sim* s, int profile, int dd, int strain, int kval_sum) {
     if (kval_sum == 0) kval_sum = 1;
+#if OR_PLANT == 4
+    kval_sum += 1;
+#endif
+#if OR_PLANT == 2
+    const int flat = profile * CULMINATION + dd + 1;
+    const float alpha = (&s->shed[0][0])[flat < 6 * CULMINATION ? flat : 6 * CULMINATION - 1];
+#else
     const float alpha = s->shed[profile][dd];
+#endif
     const float avg = fmul(alpha, s->factor[strain]);
     return fdiv(avg, (float)kval_sum);
 }
@@ -487,7 +519,11 @@ uint8_t infection_profile(const or_sim* s, uint32_t r) {
     uint32_t prof = r % 3u;
     prof += ST_INFECTED;
     const float y = unit_float(r);
+#if OR_PLANT == 1
+    if (y >= s->pct) prof += 3;
+#else
     if (y > s->pct) prof += 3;
+#endif
     return (uint8_t)prof;
 }
 
@@ -519,18 +555,39 @@ void process_contacts(or_sim* s, int32_t idx, int kval_sum, const int32_t* vic, 
         const Words4 w = draw4(M2 + (uint64_t)k, s->k0, s->k1);
         for (int j = 0; j < 4; ++j) u[4 * k + j] = w.w[j];
     }
+#if OR_PLANT == 6
+    if (gp != GEN_NONE) gp = (uint8_t)(gp - 1);
+    if (gs != GEN_NONE) gs = (uint8_t)(gs - 1);
+#endif
     const uint8_t day_to_set = (uint8_t)(d + 1);
-    int used = 0;
-    for (int c = 0; c < C; ++c) {
+    for (int k = 0; k < C; ++k) {
+#if OR_PLANT == 7
+        const int c = C - 1 - k;
+#else
+        const int c = k;
+#endif
+        int used = 2 * c; /* the reference's running rand_vals_used */
         const int32_t victim = vic[c];
         const int kv = KVAL[typ[c]];
         const float yp = unit_float(u[used]);
         const bool inf_p = yp < fmul(prob_p, (float)kv);
+#if OR_PLANT == 3
+        const uint8_t pp = infection_profile(s, u[used] ^ u[(used + 2) % 16]);
+#elif OR_PLANT == 8
+        const uint8_t pp = infection_profile(s, u[used]);
+#else
         const uint8_t pp = infection_profile(s, u[used] ^ u[(used + 1) % 16]);
+#endif
         used++;
         const float ys = unit_float(u[used]);
         const bool inf_s = ys < fmul(prob_s, (float)kv);
+#if OR_PLANT == 3
+        const uint8_t ps = infection_profile(s, u[used] ^ u[(used + 2) % 16]);
+#elif OR_PLANT == 8
+        const uint8_t ps = infection_profile(s, u[used]);
+#else
         const uint8_t ps = infection_profile(s, u[used] ^ u[(used + 1) % 16]);
+#endif
         used++;
         int action = 0;
         if (inf_p && s->st_p[victim] == ST_SUSCEPTIBLE) {
@@ -550,6 +607,8 @@ void process_contacts(or_sim* s, int32_t idx, int kval_sum, const int32_t* vic, 
             s->tr_victim[trace_base + c] = victim;
             s->tr_type[trace_base + c] = typ[c];
             s->tr_action[trace_base + c] = (uint8_t)action;
+            s->tr_draw[trace_base + c] = (uint8_t)((inf_p ? 2 : 0) | (inf_s ? 1 : 0));
+            s->tr_prof[trace_base + c] = (uint8_t)(pp | (ps << 4));
         }
     }
 }
@@ -671,6 +730,8 @@ void contacts_day(or_sim* s, bool weekend) {
         s->tr_victim.assign((size_t)I * C, -1);
         s->tr_type.assign((size_t)I * C, 0);
         s->tr_action.assign((size_t)I * C, 0);
+        s->tr_draw.assign((size_t)I * C, 0);
+        s->tr_prof.assign((size_t)I * C, 0);
     }
     int32_t vic[8];
     uint8_t typ[8];
@@ -1046,6 +1107,18 @@ int or_get_trace(or_sim* s, int32_t* infector, int32_t* victim, uint8_t* type, u
     std::memcpy(action, s->tr_action.data(), n);
     return 0;
 }
+int or_get_trace_draws(or_sim* s, uint8_t* draw, uint8_t* profile) {
+    const size_t n = s->tr_draw.size();
+    if (n == 0) return 0;
+    if (draw) std::memcpy(draw, s->tr_draw.data(), n);
+    if (profile) std::memcpy(profile, s->tr_prof.data(), n);
+    return 0;
+}
+uint8_t or_infection_profile(or_sim* s, uint32_t r) { return infection_profile(s, r); }
+float or_infection_probability(or_sim* s, int profile, int dd, int strain, int kval_sum) {
+    return infection_probability(s, profile, dd, strain, kval_sum);
+}
+int or_plant(void) { return OR_PLANT; }
 
 /* Output files: output_infected_stats.csv (PandemicSim.cu:263, :2771-2818),
  * infected_max.txt (:855-866, :923-931), output_rn.csv (:2276-2297). */

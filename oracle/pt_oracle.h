@@ -144,6 +144,20 @@ int64_t or_trace_size(or_sim* s);
 int or_get_trace(or_sim* s, int32_t* infector, int32_t* victim, uint8_t* type, uint8_t* action);
 void or_enable_trace(or_sim* s, int on);
 
+/* Draws of the same slots: draw bit 1 / bit 0 = the pandemic / seasonal
+ * draw y < p * kval succeeded (PandemicSim.cu:3195-3206) whatever the
+ * victim's state; profile = the profile each draw gives (pandemic low
+ * nibble, seasonal high nibble).  Either pointer may be NULL. */
+int or_get_trace_draws(or_sim* s, uint8_t* draw, uint8_t* profile);
+
+/* device_getInfectionProfile (PandemicSim.cu:3974-3988) and
+ * device_calculateInfectionProbability (:2307-2326) with the simulation's
+ * calibrated tables (valid after or_create). */
+uint8_t or_infection_profile(or_sim* s, uint32_t r);
+float or_infection_probability(or_sim* s, int profile, int dd, int strain, int kval_sum);
+/* Which deliberate mistake this build carries (0: none; see pt_oracle.cpp). */
+int or_plant(void);
+
 /* Writes the reference's output files into dir. */
 int or_write_outputs(or_sim* s, const char* dir, const int32_t* counts, const int32_t* infected, int32_t days);
 

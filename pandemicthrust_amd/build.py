@@ -20,7 +20,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-HIP_SOURCES = ["pt_sim.hip", "pt_batch.hip", "pt_sort.hip", "pt_mlp.hip"]
+HIP_SOURCES = ["pt_sim.hip", "pt_batch.hip", "pt_sort.hip", "pt_mlp.hip", "pt_table.hip"]
 
 
 def _run(cmd, cwd=None):
@@ -51,7 +51,7 @@ def build_host():
 
 
 def build_oracle():
-    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    _run(["make", "-s", "-j8", "-C", os.path.join(ROOT, "oracle")])
 
 
 def build_all():

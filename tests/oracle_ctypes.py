@@ -39,15 +39,20 @@ def params_from_row(row):
                   float(f[5]), float(f[6]), float(f[7]), float(f[8]))
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def plant_path(k):
+    """The oracle copy with planted mistake k (OR_PLANT, oracle/pt_oracle.cpp)."""
+    return os.path.join(ROOT, "oracle", "build", "liboracle_plant%d.so" % k)
+
+
+def lib(path=LIB_PATH):
+    L = _libs.get(path)
+    if L is None:
+        if not os.path.exists(path):
             raise RuntimeError("oracle not built: run __graft_entry__.build() or make -C oracle")
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         L.or_create.restype = C.c_void_p
         L.or_create.argtypes = [C.POINTER(Params), C.c_int32]
         L.or_destroy.argtypes = [C.c_void_p]
@@ -84,8 +89,14 @@ def lib():
         L.or_trace_size.argtypes = [C.c_void_p]
         L.or_get_trace.argtypes = [C.c_void_p] + [C.c_void_p] * 4
         L.or_write_outputs.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_int32]
-        _lib = L
-    return _lib
+        L.or_get_trace_draws.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_infection_profile.restype = C.c_uint8
+        L.or_infection_profile.argtypes = [C.c_void_p, C.c_uint32]
+        L.or_infection_probability.restype = C.c_float
+        L.or_infection_probability.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.or_plant.restype = C.c_int
+        _libs[path] = L
+    return L
 
 
 def threefry2x64(c0, c1, k0, k1):
@@ -109,16 +120,17 @@ def glibc_seeds(core_seed):
 class OracleSim:
     """One oracle simulation (mirrors the reference's PandemicSim object)."""
 
-    def __init__(self, params, core_seed):
+    def __init__(self, params, core_seed, path=LIB_PATH):
         self.params = params
         self.core_seed = core_seed
-        self.h = lib().or_create(C.byref(params), core_seed)
+        self.L = lib(path)
+        self.h = self.L.or_create(C.byref(params), core_seed)
         if not self.h:
-            raise RuntimeError(lib().or_last_error().decode())
+            raise RuntimeError(self.L.or_last_error().decode())
 
     def close(self):
         if self.h:
-            lib().or_destroy(self.h)
+            self.L.or_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -126,86 +138,86 @@ class OracleSim:
 
     def set_interventions(self, iv):
         """iv: a pandemicthrust_amd.Interventions (same layout as or_interventions)."""
-        lib().or_set_interventions(self.h, C.byref(iv))
+        self.L.or_set_interventions(self.h, C.byref(iv))
 
     def setup(self):
-        if lib().or_setup(self.h) != 0:
-            raise RuntimeError(lib().or_last_error().decode())
+        if self.L.or_setup(self.h) != 0:
+            raise RuntimeError(self.L.or_last_error().decode())
 
     def step(self):
         counts = np.zeros(16, np.int32)
         inf = C.c_int32(0)
-        rc = lib().or_step(self.h, counts.ctypes.data, C.byref(inf))
+        rc = self.L.or_step(self.h, counts.ctypes.data, C.byref(inf))
         return rc, counts, inf.value
 
     def run(self, cap_days=256):
         counts = np.zeros((cap_days, 16), np.int32)
         inf = np.zeros(cap_days, np.int32)
-        d = lib().or_run(self.h, counts.ctypes.data, inf.ctypes.data, cap_days)
+        d = self.L.or_run(self.h, counts.ctypes.data, inf.ctypes.data, cap_days)
         return counts[:d].copy(), inf[:d].copy()
 
     def scalar(self, which):
-        return int(lib().or_get_scalar(self.h, which))
+        return int(self.L.or_get_scalar(self.h, which))
 
     @property
     def n_people(self):
         return self.scalar(0)
 
     def array(self, which):
-        size = int(lib().or_get_size(self.h, which))
+        size = int(self.L.or_get_size(self.h, which))
         dt = np.uint8 if which in _U8 else np.int32
         out = np.zeros(size // np.dtype(dt).itemsize, dt)
         if size:
-            rc = lib().or_get_array(self.h, which, out.ctypes.data, size)
+            rc = self.L.or_get_array(self.h, which, out.ctypes.data, size)
             assert rc == 0
         return out
 
     def type_tables(self):
         cnt = np.zeros(14, np.int32)
         off = np.zeros(14, np.int32)
-        lib().or_get_type_tables(self.h, cnt.ctypes.data, off.ctypes.data)
+        self.L.or_get_type_tables(self.h, cnt.ctypes.data, off.ctypes.data)
         return cnt, off
 
     def infection_tables(self):
         fac = np.zeros(2, np.float32)
         shed = np.zeros(60, np.float32)
-        lib().or_get_infection_tables(self.h, fac.ctypes.data, shed.ctypes.data)
+        self.L.or_get_infection_tables(self.h, fac.ctypes.data, shed.ctypes.data)
         return fac, shed.reshape(6, 10)
 
     def generation_offsets(self):
         md = self.params.max_days
         cp = np.zeros(md + 1, np.int32)
         cs = np.zeros(md + 1, np.int32)
-        lib().or_generation_offsets(self.h, cp.ctypes.data, cs.ctypes.data, md + 1)
+        self.L.or_generation_offsets(self.h, cp.ctypes.data, cs.ctypes.data, md + 1)
         return cp, cs
 
     def set_offsets(self, workplace_off, errand_off):
-        lib().or_set_offsets(self.h, workplace_off, errand_off)
+        self.L.or_set_offsets(self.h, workplace_off, errand_off)
 
     def recalc_workplace(self, idx, age):
-        return int(lib().or_recalc_workplace(self.h, idx, age))
+        return int(self.L.or_recalc_workplace(self.h, idx, age))
 
     def recalc_weekday(self, idx, age):
         d = (C.c_int32 * 2)()
-        prof = lib().or_recalc_weekday(self.h, idx, age, d)
+        prof = self.L.or_recalc_weekday(self.h, idx, age, d)
         return int(d[0]), int(d[1]), int(prof)
 
     def recalc_weekend(self, idx):
         d = (C.c_int32 * 3)()
-        lib().or_recalc_weekend(self.h, idx, d)
+        self.L.or_recalc_weekend(self.h, idx, d)
         return int(d[0]), int(d[1]), int(d[2])
 
     def fish_workplace(self, r):
-        return int(lib().or_fish_workplace(self.h, r))
+        return int(self.L.or_fish_workplace(self.h, r))
 
     def fish_weekday_dest(self, r, age):
-        L = lib()
+        L = self.L
         L.or_fish_weekday_dest.restype = C.c_int32
         L.or_fish_weekday_dest.argtypes = [C.c_void_p, C.c_uint32, C.c_uint8]
         return int(L.or_fish_weekday_dest(self.h, r, age))
 
     def fish_weekend_dest(self, r):
-        L = lib()
+        L = self.L
         L.or_fish_weekend_dest.restype = C.c_int32
         L.or_fish_weekend_dest.argtypes = [C.c_void_p, C.c_uint32]
         return int(L.or_fish_weekend_dest(self.h, r))
@@ -213,26 +225,42 @@ class OracleSim:
     def fish_school_and_age(self, r):
         age = C.c_uint8(0)
         school = C.c_int32(0)
-        lib().or_fish_school_and_age(self.h, r, C.byref(age), C.byref(school))
+        self.L.or_fish_school_and_age(self.h, r, C.byref(age), C.byref(school))
         return int(age.value), int(school.value)
 
     def enable_trace(self, on=True):
-        lib().or_enable_trace(self.h, 1 if on else 0)
+        self.L.or_enable_trace(self.h, 1 if on else 0)
 
     def trace(self):
-        n = int(lib().or_trace_size(self.h))
+        n = int(self.L.or_trace_size(self.h))
         inf = np.zeros(n, np.int32)
         vic = np.zeros(n, np.int32)
         typ = np.zeros(n, np.uint8)
         act = np.zeros(n, np.uint8)
         if n:
-            lib().or_get_trace(self.h, inf.ctypes.data, vic.ctypes.data, typ.ctypes.data, act.ctypes.data)
+            self.L.or_get_trace(self.h, inf.ctypes.data, vic.ctypes.data, typ.ctypes.data, act.ctypes.data)
         return inf, vic, typ, act
+
+    def trace_draws(self):
+        """Per slot of the last contact day: draw bits (2 pandemic, 1 seasonal
+        succeeded) and profile draws (low nibble pandemic, high nibble seasonal)."""
+        n = int(self.L.or_trace_size(self.h))
+        draw = np.zeros(n, np.uint8)
+        prof = np.zeros(n, np.uint8)
+        if n:
+            self.L.or_get_trace_draws(self.h, draw.ctypes.data, prof.ctypes.data)
+        return draw, prof
+
+    def infection_profile(self, r):
+        return int(self.L.or_infection_profile(self.h, r))
+
+    def infection_probability(self, profile, dd, strain, kval_sum):
+        return float(self.L.or_infection_probability(self.h, profile, dd, strain, kval_sum))
 
     def write_outputs(self, directory, counts, infected):
         counts = np.ascontiguousarray(counts, np.int32)
         infected = np.ascontiguousarray(infected, np.int32)
-        rc = lib().or_write_outputs(self.h, directory.encode(), counts.ctypes.data, infected.ctypes.data,
+        rc = self.L.or_write_outputs(self.h, directory.encode(), counts.ctypes.data, infected.ctypes.data,
                                     len(infected))
         assert rc == 0
 
