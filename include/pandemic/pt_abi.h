@@ -226,7 +226,7 @@ int pt_device_name(int32_t device, char* buf, int32_t len);
  * (household, workplace or school, after-school location, errands as
  * hour * W + location, -1 for none) and the action (ACTION_INFECT_*,
  * simParameters.h:23-26: 2 pandemic, 1 seasonal, 3 both, for the attempts
- * that infected the victim in the canonical order).  Costs 10 bytes per slot
+ * that infected the victim in the canonical order).  Costs 12 bytes per slot
  * of device memory (8N slots); results are unchanged.  pt_sim_trace copies
  * the last day's pt_sim_trace_size entries; infector = the infected agent of
  * the slot's position.  Returns the number of entries. */
@@ -234,6 +234,15 @@ int pt_sim_enable_trace(pt_sim* sim, int32_t on);
 int64_t pt_sim_trace_size(pt_sim* sim);
 int pt_sim_trace(pt_sim* sim, int32_t* infector, int32_t* victim, uint8_t* type, int32_t* loc, uint8_t* action,
                  int64_t n);
+/* The draws of the same slots, as the reference's validation mode records
+ * its uniforms and thresholds (rand_arr_1..4, PandemicSim.cu:3220-3230):
+ * draw bit 1 / bit 0 = the pandemic / seasonal draw succeeded (y < p * kval,
+ * :3195-3206), whatever the victim's state; profile = the infection profile
+ * each draw gives (device_getInfectionProfile, :3974-3988; pandemic in the
+ * low nibble, seasonal in the high nibble), computed for every slot as the
+ * reference does.  Either pointer may be NULL.  Returns the number of
+ * entries (pt_sim_trace_size). */
+int pt_sim_trace_draws(pt_sim* sim, uint8_t* draw, uint8_t* profile, int64_t n);
 
 /* Launch shapes and table-strategy thresholds.  They change only speed:
  * results are identical for every accepted value (the kernels stride over
@@ -261,6 +270,20 @@ enum pt_launch_knob {
 };
 int pt_sim_set_launch(pt_sim* sim, int32_t knob, int32_t value);
 int pt_sim_get_launch(pt_sim* sim, int32_t knob);
+
+/* Location table on caller-owned device memory: the reference's daily
+ * cub::DeviceRadixSort::SortPairs + thrust::lower_bound
+ * (PandemicSim.cu:1212-1248, :1316-1354; workplace table :3835-3862) as one
+ * call.  keys[n]: uint32 bucket keys, each < n_buckets, of pairs whose ids
+ * are their positions 0..n-1.  Writes people[n] = the ids grouped by key,
+ * ascending within a key (the stable sort's order), and
+ * offsets[n_buckets + 1] with offsets[k] = number of keys below k (so
+ * offsets[n_buckets] = n).  All three are device pointers on `device`;
+ * stream is a hipStream_t (NULL: the default stream).  The radix scratch
+ * (about 20 bytes per pair) is allocated for the call and freed before it
+ * returns, so the call synchronises the stream.  n < 2^31, n_buckets >= 1. */
+int pt_location_table(const uint32_t* keys, int64_t n, int32_t n_buckets, int32_t* people, int32_t* offsets,
+                      int32_t device, void* stream);
 
 /* Blocks until the handle's stream is idle; loop time of the last
  * pt_sim_run in milliseconds (HIP events around the enqueued days). */

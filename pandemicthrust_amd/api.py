@@ -35,7 +35,7 @@ EXPORTS = [
     "pt_sim_day_kernel_times", "pt_sim_setup_times", "pt_sim_daily_rows", "pt_sim_checkpoint_bytes",
     "pt_sim_checkpoint_save", "pt_sim_checkpoint_load", "pt_device_memory", "pt_device_name",
     "pt_batch_use_graphs", "pt_sim_set_launch", "pt_sim_get_launch", "pt_batch_set_launch", "pt_batch_get_launch",
-    "pt_sim_enable_trace", "pt_sim_trace_size", "pt_sim_trace",
+    "pt_sim_enable_trace", "pt_sim_trace_size", "pt_sim_trace", "pt_sim_trace_draws", "pt_location_table",
 ]
 
 # enum pt_launch_knob (include/pandemic/pt_abi.h)
@@ -170,6 +170,9 @@ def lib():
         L.pt_sim_trace_size.restype = C.c_int64
         L.pt_sim_trace_size.argtypes = [C.c_void_p]
         L.pt_sim_trace.argtypes = [C.c_void_p] + [C.c_void_p] * 5 + [C.c_int64]
+        L.pt_sim_trace_draws.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        L.pt_location_table.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
+                                        C.c_void_p]
         for f in ("pt_sim_set_launch", "pt_batch_set_launch"):
             getattr(L, f).argtypes = [C.c_void_p, C.c_int32, C.c_int32]
         for f in ("pt_sim_get_launch", "pt_batch_get_launch"):
@@ -298,6 +301,19 @@ class PandemicSim:
             _check(lib().pt_sim_trace(self.h, out["infector"].ctypes.data, out["victim"].ctypes.data,
                                       out["type"].ctypes.data, out["loc"].ctypes.data, out["action"].ctypes.data, n))
         return out
+
+    def trace_draws(self):
+        """The last contact day's draws per slot (pt_sim_trace_draws): draw bits
+        (2 pandemic, 1 seasonal succeeded) and profile draws (low nibble
+        pandemic, high nibble seasonal)."""
+        n = int(lib().pt_sim_trace_size(self.h))
+        if n < 0:
+            raise PTError(lib().pt_last_error().decode())
+        draw = np.zeros(n, np.uint8)
+        prof = np.zeros(n, np.uint8)
+        if n:
+            _check(lib().pt_sim_trace_draws(self.h, draw.ctypes.data, prof.ctypes.data, n))
+        return draw, prof
 
     def set_launch(self, name, value):
         """Launch shape (LAUNCH_KNOBS); speed only, results unchanged."""

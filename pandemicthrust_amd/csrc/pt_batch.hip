@@ -94,6 +94,8 @@ __device__ __forceinline__ AgentArrays view(const BatchArrays& B, const RepInfo&
     A.trace_type = nullptr;
     A.trace_loc = nullptr;
     A.trace_attempt = nullptr;
+    A.trace_draw = nullptr;
+    A.trace_prof = nullptr;
     return A;
 }
 

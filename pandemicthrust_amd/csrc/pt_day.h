@@ -28,6 +28,25 @@ int fail(int code, const char* fmt, ...);
 // PT_OK when value is in range for launch setting knob (pt_launch_knob).
 int check_launch(int32_t knob, int32_t value);
 
+// Argument checks shared by the location-table entry points (pt_table.hip).
+int location_table_args(const uint32_t* keys, int64_t n, int32_t n_buckets, const int32_t* people,
+                        const int32_t* offsets);
+
+// Makes `device` current for a scope and restores the caller's device.
+struct DeviceScope {
+    int saved = -1;
+    hipError_t err = hipSuccess;
+    explicit DeviceScope(int device) {
+        err = hipGetDevice(&saved);
+        if (err == hipSuccess) err = hipSetDevice(device);
+    }
+    ~DeviceScope() {
+        if (saved >= 0) (void)hipSetDevice(saved);
+    }
+    DeviceScope(const DeviceScope&) = delete;
+    DeviceScope& operator=(const DeviceScope&) = delete;
+};
+
 #define HIP_TRY(expr)                                                                              \
     do {                                                                                           \
         hipError_t e_ = (expr);                                                                    \
@@ -504,6 +523,14 @@ struct AgentArrays {
     uint8_t* trace_type;
     int32_t* trace_loc;
     uint8_t* trace_attempt;
+    // Optional per-slot draw record (validation mode), [I * C]: trace_draw
+    // bit 1 / bit 0 = the pandemic / seasonal draw y < p * kval succeeded,
+    // whatever the victim's state; trace_prof = the profile each strain's
+    // draw would give (device_getInfectionProfile, PandemicSim.cu:3974-3988),
+    // pandemic in the low nibble, seasonal in the high nibble.  The
+    // reference computes both for every slot (PandemicSim.cu:3195-3211).
+    uint8_t* trace_draw;
+    uint8_t* trace_prof;
 };
 
 // Strain state of an infector (device_processContacts, PandemicSim.cu:3125-3175):
@@ -570,7 +597,15 @@ __device__ __forceinline__ void process_contacts(const Model& M, const AgentArra
         const float kv = typ[c] == CT_NONE ? 0.f : 1.f;
         const int32_t v = vic[c];
         uint8_t attempt = 0;
-        if (unit_float(u[2 * c]) < prob_p * kv) {
+        const bool draw_p = unit_float(u[2 * c]) < prob_p * kv;
+        const bool draw_s = unit_float(u[2 * c + 1]) < prob_s * kv;
+        if (A.trace_draw) {
+            const int64_t e = (int64_t)pos * C + c;
+            A.trace_draw[e] = (uint8_t)((draw_p ? 2 : 0) | (draw_s ? 1 : 0));
+            A.trace_prof[e] = (uint8_t)(infection_profile(M, u[2 * c] ^ u[(2 * c + 1) % 16]) |
+                                        (infection_profile(M, u[2 * c + 1] ^ u[(2 * c + 2) % 16]) << 4));
+        }
+        if (draw_p) {
             if (A.st_p[v] == kSusceptible) {
                 const uint32_t prof = infection_profile(M, u[2 * c] ^ u[(2 * c + 1) % 16]);
                 const uint64_t claim = ((uint64_t)(pos * 8u + (uint32_t)c) << 32) | ((uint32_t)f.gen_p << 8) | prof;
@@ -579,7 +614,7 @@ __device__ __forceinline__ void process_contacts(const Model& M, const AgentArra
                 attempt |= 2;
             }
         }
-        if (unit_float(u[2 * c + 1]) < prob_s * kv) {
+        if (draw_s) {
             if (A.st_s[v] == kSusceptible) {
                 const uint32_t prof = infection_profile(M, u[2 * c + 1] ^ u[(2 * c + 2) % 16]);
                 const uint64_t claim = ((uint64_t)(pos * 8u + (uint32_t)c) << 32) | ((uint32_t)f.gen_s << 8) | prof;
@@ -699,7 +734,10 @@ __device__ __forceinline__ void weekday_contacts(const Model& M, const AgentArra
             A.trace_loc[pos * 8 + c] = vic[c] < 0 ? -1 : where[c];
         }
     }
-    if (can_infect(M, f))
+    // Traced days run the draws of every infector, as the reference does: an
+    // infector that cannot infect has p <= 0 (or NaN), so none of its draws
+    // succeeds and it posts no claim.
+    if (can_infect(M, f) || A.trace_draw)
         process_contacts<8>(M, A, f, kval, vic, typ, Mc + 2, (uint32_t)pos, newp, news);
     else if (A.trace_attempt)
         for (int c = 0; c < 8; ++c) A.trace_attempt[pos * 8 + c] = 0;
@@ -748,7 +786,10 @@ __device__ __forceinline__ void weekend_contacts(const Model& M, const AgentArra
             A.trace_loc[pos * 5 + c] = vic[c] < 0 ? -1 : where[c];
         }
     }
-    if (can_infect(M, f))
+    // Traced days run the draws of every infector, as the reference does: an
+    // infector that cannot infect has p <= 0 (or NaN), so none of its draws
+    // succeeds and it posts no claim.
+    if (can_infect(M, f) || A.trace_draw)
         process_contacts<5>(M, A, f, kval, vic, typ, Mc + 2, (uint32_t)pos, newp, news);
     else if (A.trace_attempt)
         for (int c = 0; c < 5; ++c) A.trace_attempt[pos * 5 + c] = 0;

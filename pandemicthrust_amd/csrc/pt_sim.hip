@@ -832,7 +832,7 @@ struct pt_sim {
     // Validation trace of the last contact day (pt_sim_enable_trace).
     bool trace_on = false;
     int32_t *tr_victim = nullptr, *tr_loc = nullptr;
-    uint8_t *tr_type = nullptr, *tr_attempt = nullptr, *tr_action = nullptr;
+    uint8_t *tr_type = nullptr, *tr_attempt = nullptr, *tr_action = nullptr, *tr_draw = nullptr, *tr_prof = nullptr;
     int cus = 128;                                   // compute units of the device
     int launch[PT_LAUNCH_KNOBS] = {8, 16, 8, 100, 75, 8, 1};  // pt_launch_knob values
 };
@@ -855,13 +855,16 @@ void free_trace(pt_sim* s) {
     dev_free(s->tr_loc);
     dev_free(s->tr_attempt);
     dev_free(s->tr_action);
+    dev_free(s->tr_draw);
+    dev_free(s->tr_prof);
 }
 
 int alloc_trace(pt_sim* s) {
     free_trace(s);
     const int64_t n = 8 * s->N;
     if (dev_alloc(&s->tr_victim, n) || dev_alloc(&s->tr_type, n) || dev_alloc(&s->tr_loc, n) ||
-        dev_alloc(&s->tr_attempt, n) || dev_alloc(&s->tr_action, n)) {
+        dev_alloc(&s->tr_attempt, n) || dev_alloc(&s->tr_action, n) || dev_alloc(&s->tr_draw, n) ||
+        dev_alloc(&s->tr_prof, n)) {
         free_trace(s);
         return PT_ENOMEM;
     }
@@ -927,6 +930,8 @@ AgentArrays agent_arrays(pt_sim* s) {
     A.trace_type = s->trace_on ? s->tr_type : nullptr;
     A.trace_loc = s->trace_on ? s->tr_loc : nullptr;
     A.trace_attempt = s->trace_on ? s->tr_attempt : nullptr;
+    A.trace_draw = s->trace_on ? s->tr_draw : nullptr;
+    A.trace_prof = s->trace_on ? s->tr_prof : nullptr;
     return A;
 }
 
@@ -1749,6 +1754,19 @@ int pt_sim_trace(pt_sim* s, int32_t* infector, int32_t* victim, uint8_t* type, i
     if (type) HIP_TRY(hipMemcpy(type, s->tr_type, (size_t)have, hipMemcpyDeviceToHost));
     if (loc) HIP_TRY(hipMemcpy(loc, s->tr_loc, (size_t)have * 4, hipMemcpyDeviceToHost));
     if (action) HIP_TRY(hipMemcpy(action, s->tr_action, (size_t)have, hipMemcpyDeviceToHost));
+    return (int)std::min<int64_t>(have, 0x7FFFFFFF);
+}
+
+int pt_sim_trace_draws(pt_sim* s, uint8_t* draw, uint8_t* profile, int64_t n) {
+    if (!s) return fail(PT_EINVAL, "null handle");
+    const int64_t have = pt_sim_trace_size(s);
+    if (have < 0) return PT_EDEVICE;
+    if (n < have) return fail(PT_EINVAL, "trace holds %lld entries, room for %lld", (long long)have, (long long)n);
+    if (have == 0) return 0;
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (draw) HIP_TRY(hipMemcpy(draw, s->tr_draw, (size_t)have, hipMemcpyDeviceToHost));
+    if (profile) HIP_TRY(hipMemcpy(profile, s->tr_prof, (size_t)have, hipMemcpyDeviceToHost));
     return (int)std::min<int64_t>(have, 0x7FFFFFFF);
 }
 
