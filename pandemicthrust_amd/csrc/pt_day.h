@@ -492,6 +492,48 @@ __device__ __forceinline__ int select_person(int32_t infector, int32_t loc_off, 
     return CT_NONE;
 }
 
+// Where the victims of an infector's contact slots come from: the household
+// (slots 0..2; its members are consecutive ids), then `wpd` workplace or
+// school slots, then h0 slots in the first errand location and h1 in the
+// second (weekend: wpd = 0, h0 = h1 = 1), with each slot's draw r.  Only a
+// slot whose infection draw succeeds needs its victim, so outside
+// validation mode the contact kernels look victims up then (pick) instead of
+// gathering one from a location table for every slot: the contact types and
+// kval_sum depend on the location counts alone (select_person), and the
+// victims of unsuccessful slots are never read.
+struct SlotPlan {
+    int32_t hoff, hcnt, woff, wcnt, e0off, e0cnt, e1off, e1cnt;
+    int wpd, h0, h1;
+    uint32_t r[8];
+};
+
+// The victim of slot c (select_person's pick; the slot's count is > 1).
+__device__ __forceinline__ int32_t pick(const SlotPlan& P, const int32_t* __restrict__ wp_people,
+                                        const int32_t* __restrict__ er_people, int32_t infector, int c) {
+    int32_t off, cnt;
+    const int32_t* people;
+    if (c < 3) {
+        off = P.hoff;
+        cnt = P.hcnt;
+        people = nullptr;
+    } else if (c < 3 + P.wpd) {
+        off = P.woff;
+        cnt = P.wcnt;
+        people = wp_people;
+    } else if (c < 3 + P.wpd + P.h0) {
+        off = P.e0off;
+        cnt = P.e0cnt;
+        people = er_people;
+    } else {
+        off = P.e1off;
+        cnt = P.e1cnt;
+        people = er_people;
+    }
+    int32_t v;
+    select_person(infector, off, cnt, P.r[c], people ? CT_ERRAND : CT_HOME, people, v);
+    return v;
+}
+
 // The arrays of one simulation in its local numbering.  er_off is indexed by
 // the simulation's bucket key; its values (and the other offsets) index the
 // corresponding people arrays as given.
@@ -579,10 +621,13 @@ __device__ __forceinline__ bool can_infect(const Model& M, const Infectivity& f)
 // Every successful attempt on a victim that is susceptible at the start of
 // the day posts ((pos*8 + slot) << 32 | gen << 8 | profile) with a 64-bit
 // atomicMin into the victim's claim word; the first poster is told so.
+// vic: the slots' victims when `resolved` (validation mode), else filled
+// here for the slots whose draw succeeds (pick).
 template <int C>
 __device__ __forceinline__ void process_contacts(const Model& M, const AgentArrays& A, const Infectivity& f, int kval_sum,
-                                                 const int32_t (&vic)[8], const int (&typ)[8], uint64_t M2,
-                                                 uint32_t pos, bool (&newp)[8], bool (&news)[8]) {
+                                                 int32_t (&vic)[8], const int (&typ)[8], const SlotPlan& P,
+                                                 bool resolved, int32_t idx, uint64_t M2, uint32_t pos,
+                                                 bool (&newp)[8], bool (&news)[8]) {
     const float prob_p = f.prof_p >= 0 ? infection_probability(M, f.prof_p, f.dd_p, 0, kval_sum) : -1.f;
     const float prob_s = f.prof_s >= 0 ? infection_probability(M, f.prof_s, f.dd_s, 1, kval_sum) : -1.f;
     uint32_t u[16];
@@ -595,10 +640,14 @@ __device__ __forceinline__ void process_contacts(const Model& M, const AgentArra
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const float kv = typ[c] == CT_NONE ? 0.f : 1.f;
-        const int32_t v = vic[c];
         uint8_t attempt = 0;
         const bool draw_p = unit_float(u[2 * c]) < prob_p * kv;
         const bool draw_s = unit_float(u[2 * c + 1]) < prob_s * kv;
+        int32_t v = vic[c];
+        if (!resolved && (draw_p || draw_s)) {  // kv = 1: a real contact, count > 1
+            v = pick(P, A.wp_people, A.er_people, idx, c);
+            vic[c] = v;
+        }
         if (A.trace_draw) {
             const int64_t e = (int64_t)pos * C + c;
             A.trace_draw[e] = (uint8_t)((draw_p ? 2 : 0) | (draw_s ? 1 : 0));
@@ -697,30 +746,41 @@ __device__ __forceinline__ void weekday_contacts(const Model& M, const AgentArra
     const int32_t e1off = A.er_off[k1], e1cnt = A.er_off[k1 + 1] - e1off;
     const int wtype = adult ? CT_WORKPLACE : CT_SCHOOL;
     const int etype = adult ? CT_ERRAND : CT_AFTERSCHOOL;
+    const SlotPlan P{hoff, hcnt, woff, wcnt, e0off, e0cnt, e1off, e1cnt, wpd, h0, h1,
+                     {u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7]}};
+    const bool resolved = A.trace_victim != nullptr;
     int kval = 0;
     int32_t where[8];  // the contact's location (trace only)
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         int t;
-        if (c < 3) {
-            t = select_person(idx, hoff, hcnt, u[c], CT_HOME, nullptr, vic[c]);
-            where[c] = hh;
-        } else if (c < 3 + wpd) {
-            t = select_person(idx, woff, wcnt, u[c], wtype, A.wp_people, vic[c]);
-            where[c] = wp;
-        } else {
-            const int e = c - 3 - wpd;
-            if (e < h0) {
-                t = select_person(idx, e0off, e0cnt, u[c], etype, A.er_people, vic[c]);
-                where[c] = d1;
-            } else if (e < h0 + h1) {
-                t = select_person(idx, e1off, e1cnt, u[c], etype, A.er_people, vic[c]);
-                where[c] = adult ? M.W + d2 : d2;
+        if (resolved) {
+            if (c < 3) {
+                t = select_person(idx, hoff, hcnt, u[c], CT_HOME, nullptr, vic[c]);
+                where[c] = hh;
+            } else if (c < 3 + wpd) {
+                t = select_person(idx, woff, wcnt, u[c], wtype, A.wp_people, vic[c]);
+                where[c] = wp;
             } else {
-                vic[c] = -1;
-                t = CT_NONE;
-                where[c] = -1;
+                const int e = c - 3 - wpd;
+                if (e < h0) {
+                    t = select_person(idx, e0off, e0cnt, u[c], etype, A.er_people, vic[c]);
+                    where[c] = d1;
+                } else if (e < h0 + h1) {
+                    t = select_person(idx, e1off, e1cnt, u[c], etype, A.er_people, vic[c]);
+                    where[c] = adult ? M.W + d2 : d2;
+                } else {
+                    vic[c] = -1;
+                    t = CT_NONE;
+                    where[c] = -1;
+                }
             }
+        } else {
+            // select_person's type without the pick
+            const int e = c - 3 - wpd;
+            const int32_t cnt = c < 3 ? hcnt : (c < 3 + wpd ? wcnt : (e < h0 ? e0cnt : (e < h0 + h1 ? e1cnt : 0)));
+            const int ty = c < 3 ? CT_HOME : (c < 3 + wpd ? wtype : etype);
+            t = (e < h0 + h1 && cnt > 1) ? ty : CT_NONE;
         }
         typ[c] = t;
         kval += t != CT_NONE;
@@ -738,7 +798,7 @@ __device__ __forceinline__ void weekday_contacts(const Model& M, const AgentArra
     // infector that cannot infect has p <= 0 (or NaN), so none of its draws
     // succeeds and it posts no claim.
     if (can_infect(M, f) || A.trace_draw)
-        process_contacts<8>(M, A, f, kval, vic, typ, Mc + 2, (uint32_t)pos, newp, news);
+        process_contacts<8>(M, A, f, kval, vic, typ, P, resolved, idx, Mc + 2, (uint32_t)pos, newp, news);
     else if (A.trace_attempt)
         for (int c = 0; c < 8; ++c) A.trace_attempt[pos * 8 + c] = 0;
 }
@@ -755,14 +815,21 @@ __device__ __forceinline__ void weekend_contacts(const Model& M, const AgentArra
     const int32_t hh = A.household[idx];
     const int32_t hoff = A.hh_off[hh];
     const int32_t hcnt = A.hh_off[hh + 1] - hoff;
+    const bool resolved = A.trace_victim != nullptr;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) typ[c] = select_person(idx, hoff, hcnt, u.w[c], CT_HOME, nullptr, vic[c]);
+    for (int c = 0; c < 3; ++c) {
+        if (resolved)
+            typ[c] = select_person(idx, hoff, hcnt, u.w[c], CT_HOME, nullptr, vic[c]);
+        else
+            typ[c] = hcnt > 1 ? CT_HOME : CT_NONE;
+    }
     const int prof = (int)(u.w[3] % 6u);
     uint32_t v32[2];
     draw2x32((uint32_t)(Mc + 1), (uint32_t)M.key0, (uint32_t)M.key1, v32);
     int32_t hour[3], loc[3];
     weekend_errands(M, eoff + (uint64_t)(int64_t)(2 * idx), hour, loc);
     int32_t where[5] = {hh, hh, hh, -1, -1};  // the contact's location (trace only)
+    int32_t eo[2], ec[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int slot = M.we_assign[prof][k];
@@ -770,10 +837,16 @@ __device__ __forceinline__ void weekend_contacts(const Model& M, const AgentArra
         const int lsel = slot == 0 ? loc[0] : (slot == 1 ? loc[1] : loc[2]);
         const int32_t l = min(max(lsel - M.we_base, 0), M.we_E - 1);
         const int32_t key = hsel * M.we_E + l;
-        const int32_t eoff2 = A.er_off[key], ecnt = A.er_off[key + 1] - eoff2;
-        typ[3 + k] = select_person(idx, eoff2, ecnt, v32[k], CT_ERRAND, A.er_people, vic[3 + k]);
+        eo[k] = A.er_off[key];
+        ec[k] = A.er_off[key + 1] - eo[k];
+        if (resolved)
+            typ[3 + k] = select_person(idx, eo[k], ec[k], v32[k], CT_ERRAND, A.er_people, vic[3 + k]);
+        else
+            typ[3 + k] = ec[k] > 1 ? CT_ERRAND : CT_NONE;
         where[3 + k] = hsel * M.W + lsel;
     }
+    const SlotPlan P{hoff, hcnt, 0, 0, eo[0], ec[0], eo[1], ec[1], 0, 1, 1,
+                     {u.w[0], u.w[1], u.w[2], v32[0], v32[1], 0u, 0u, 0u}};
     int kval = 0;
 #pragma unroll
     for (int c = 0; c < 5; ++c) kval += typ[c] != CT_NONE;
@@ -790,7 +863,7 @@ __device__ __forceinline__ void weekend_contacts(const Model& M, const AgentArra
     // infector that cannot infect has p <= 0 (or NaN), so none of its draws
     // succeeds and it posts no claim.
     if (can_infect(M, f) || A.trace_draw)
-        process_contacts<5>(M, A, f, kval, vic, typ, Mc + 2, (uint32_t)pos, newp, news);
+        process_contacts<5>(M, A, f, kval, vic, typ, P, resolved, idx, Mc + 2, (uint32_t)pos, newp, news);
     else if (A.trace_attempt)
         for (int c = 0; c < 5; ++c) A.trace_attempt[pos * 5 + c] = 0;
 }
