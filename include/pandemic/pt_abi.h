@@ -285,6 +285,19 @@ int pt_sim_get_launch(pt_sim* sim, int32_t knob);
 int pt_location_table(const uint32_t* keys, int64_t n, int32_t n_buckets, int32_t* people, int32_t* offsets,
                       int32_t device, void* stream);
 
+/* The same table, allocating nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The radix
+ * scratch is a file-scope device array of the library, one copy per device
+ * (about 85 MB), holding at most PT_LOCATION_TABLE_NOALLOC_MAX_PAIRS pairs;
+ * larger n is refused with PT_EINVAL.  The call only enqueues work on
+ * stream and returns; results are identical to pt_location_table's.  Every
+ * call on one device shares that device's array, so calls that overlap on
+ * one device (other streams or threads) race: serialise them. */
+#define PT_LOCATION_TABLE_NOALLOC_MAX_PAIRS 4194304
+int pt_location_table_noalloc(const uint32_t* keys, int64_t n, int32_t n_buckets, int32_t* people,
+                              int32_t* offsets, int32_t device, void* stream);
+
 /* Blocks until the handle's stream is idle; loop time of the last
  * pt_sim_run in milliseconds (HIP events around the enqueued days). */
 int pt_sim_synchronize(pt_sim* sim);

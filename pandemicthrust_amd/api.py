@@ -36,7 +36,11 @@ EXPORTS = [
     "pt_sim_checkpoint_save", "pt_sim_checkpoint_load", "pt_device_memory", "pt_device_name",
     "pt_batch_use_graphs", "pt_sim_set_launch", "pt_sim_get_launch", "pt_batch_set_launch", "pt_batch_get_launch",
     "pt_sim_enable_trace", "pt_sim_trace_size", "pt_sim_trace", "pt_sim_trace_draws", "pt_location_table",
+    "pt_location_table_noalloc",
 ]
+
+# include/pandemic/pt_abi.h: largest input of pt_location_table_noalloc
+PT_LOCATION_TABLE_NOALLOC_MAX_PAIRS = 4194304
 
 # enum pt_launch_knob (include/pandemic/pt_abi.h)
 LAUNCH_KNOBS = ["contact_blocks_per_cu", "gen_blocks_per_cu", "sort_wide_bits", "mark_limit_pct", "subset_share_pct",
@@ -173,6 +177,7 @@ def lib():
         L.pt_sim_trace_draws.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         L.pt_location_table.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
                                         C.c_void_p]
+        L.pt_location_table_noalloc.argtypes = L.pt_location_table.argtypes
         for f in ("pt_sim_set_launch", "pt_batch_set_launch"):
             getattr(L, f).argtypes = [C.c_void_p, C.c_int32, C.c_int32]
         for f in ("pt_sim_get_launch", "pt_batch_get_launch"):

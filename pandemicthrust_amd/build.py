@@ -20,7 +20,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-HIP_SOURCES = ["pt_sim.hip", "pt_batch.hip", "pt_sort.hip", "pt_mlp.hip", "pt_table.hip"]
+HIP_SOURCES = ["pt_sim.hip", "pt_batch.hip", "pt_sort.hip", "pt_mlp.hip", "pt_table.hip", "pt_table_noalloc.hip"]
 
 
 def _run(cmd, cwd=None):
