@@ -28,6 +28,13 @@ extern "C" {
 
 typedef struct pt_batch pt_batch;
 
+/* Capacity: replicate r occupies its population rounded up to a multiple
+ * of PT_BATCH_SLOT_TILE agent slots, and a batch holds at most
+ * PT_BATCH_MAX_SLOTS slots in all (agent ids are int32 and a weekend's 3 pairs
+ * per agent are counted in 32 bits); pt_batch_setup refuses larger batches. */
+#define PT_BATCH_SLOT_TILE 8192
+#define PT_BATCH_MAX_SLOTS 715827882
+
 /* n replicates: params[r] and seed_keys[2r], seed_keys[2r+1] (the Threefry
  * key of replicate r, SEED[0..1] of setup_loadSeed PandemicSim.cu:2843-2862)
  * as pt_sim_create takes them. */

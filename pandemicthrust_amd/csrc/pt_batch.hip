@@ -563,6 +563,10 @@ struct pt_batch {
     int cus = 256;                                   // compute units of the device
     int launch[PT_LAUNCH_KNOBS] = {8, 16, 8, 100, 75, 9, 1};  // pt_launch_knob values
     hipEvent_t run_begin = nullptr, run_end = nullptr;
+    // pt_batch_run's early stop: the batch's `done` flag copied back after
+    // each group of days (pinned, two slots) and the groups' events.
+    int32_t* h_done = nullptr;
+    hipEvent_t group_ev[2] = {nullptr, nullptr};
     // The day step captured once per layout as HIP graphs (weekday,
     // weekend) and replayed for every day; the kernels take the day and
     // every count from the device, so one capture serves all days.
@@ -825,6 +829,12 @@ int pt_batch_create(int32_t n, const pt_params* params, const int32_t* seed_keys
         pt_batch_destroy(b);
         return fail(PT_ENOMEM, "%s", msg.c_str());
     }
+    for (auto& e : b->group_ev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    if (hipHostMalloc((void**)&b->h_done, 2 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+        b->h_done = nullptr;
+        pt_batch_destroy(b);
+        return fail(PT_ENOMEM, "pinned host allocation failed");
+    }
     (void)hipEventCreate(&b->run_begin);
     (void)hipEventCreate(&b->run_end);
     *out = b;
@@ -892,9 +902,10 @@ int pt_batch_setup(pt_batch* b, int64_t* n_people_out) {
         b->rand_after[r] = ri.workplace_off + (uint64_t)(N / 4);
     }
     // Global agent ids are int32 and pair counts uint32.
-    if (P > 0x7FFFFFFF / 3 || kwd > 0x7FFFFFFF || kwe > 0x7FFFFFFF)
+    static_assert(PT_BATCH_MAX_SLOTS == 0x7FFFFFFF / 3 && PT_BATCH_SLOT_TILE == kSweepTile, "pt_batch.h limits");
+    if (P > PT_BATCH_MAX_SLOTS || kwd > 0x7FFFFFFF || kwe > 0x7FFFFFFF)
         return fail(PT_EINVAL, "batch of %lld agent slots exceeds the supported total (%d)", (long long)P,
-                    0x7FFFFFFF / 3);
+                    PT_BATCH_MAX_SLOTS);
     b->P = P;
     b->n_agents = n_agents;
     b->ntiles = tiles;
@@ -1066,11 +1077,26 @@ int pt_batch_run(pt_batch* b, int32_t* counts_out, int32_t* infected_out, int32_
     if (b->use_graphs && (!b->day_graph[0] || b->graph_table_mode != b->table_mode))
         if (capture_day_graphs(b)) return PT_EDEVICE;
     HIP_TRY(hipEventRecord(b->run_begin, b->stream));
-    for (int d = 0; d < b->D; ++d) {
-        if (b->use_graphs)
-            HIP_TRY(hipGraphLaunch(b->day_graph[(d % 7) >= 5 ? 1 : 0], b->stream));
-        else if (enqueue_batch_day(b, d))
-            return PT_EDEVICE;
+    // Days go out in groups; after each group the batch's done flag (every
+    // replicate has ended) is copied back, and once the group before the one
+    // just enqueued had ended the batch no further group is enqueued.  Days
+    // after the end are no-ops (every kernel checks the flag).
+    constexpr int kRunGroup = 4;
+    for (int d = 0, grp = 0; d < b->D; ++grp) {
+        const int end = std::min(d + kRunGroup, b->D);
+        for (; d < end; ++d) {
+            if (b->use_graphs)
+                HIP_TRY(hipGraphLaunch(b->day_graph[(d % 7) >= 5 ? 1 : 0], b->stream));
+            else if (enqueue_batch_day(b, d))
+                return PT_EDEVICE;
+        }
+        HIP_TRY(hipMemcpyAsync(&b->h_done[grp & 1], &b->d_g->done, sizeof(int32_t), hipMemcpyDeviceToHost,
+                               b->stream));
+        HIP_TRY(hipEventRecord(b->group_ev[grp & 1], b->stream));
+        if (grp > 0) {
+            HIP_TRY(hipEventSynchronize(b->group_ev[(grp - 1) & 1]));
+            if (b->h_done[(grp - 1) & 1]) break;
+        }
     }
     HIP_TRY(hipEventRecord(b->run_end, b->stream));
     HIP_TRY(hipEventSynchronize(b->run_end));
@@ -1251,6 +1277,9 @@ void pt_batch_destroy(pt_batch* b) {
     dev_free(b->d_scratch64);
     if (b->run_begin) (void)hipEventDestroy(b->run_begin);
     if (b->run_end) (void)hipEventDestroy(b->run_end);
+    for (auto e : b->group_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->h_done) (void)hipHostFree(b->h_done);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }

@@ -825,6 +825,10 @@ struct pt_sim {
     int64_t last_table_n = 0;
     EventTimer timer;
     hipEvent_t run_begin = nullptr, run_end = nullptr;
+    // pt_sim_run's early stop: the loop's `done` flag copied back after each
+    // group of days (pinned, two slots) and the groups' completion events.
+    int32_t* h_done = nullptr;
+    hipEvent_t group_ev[2] = {nullptr, nullptr};
     hipEvent_t setup_ev[6] = {};               // population size (2), build_population marks (4)
     double setup_ms[PT_SETUP_PHASES] = {0};
     double last_run_ms = 0;
@@ -1162,6 +1166,12 @@ int pt_sim_create(const pt_params* p, const int32_t seed_key[2], int32_t device,
     (void)hipEventCreate(&s->run_begin);
     (void)hipEventCreate(&s->run_end);
     for (auto& e : s->setup_ev) (void)hipEventCreate(&e);
+    for (auto& e : s->group_ev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    if (hipHostMalloc((void**)&s->h_done, 2 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+        s->h_done = nullptr;
+        pt_sim_destroy(s);
+        return fail(PT_ENOMEM, "pinned host allocation failed");
+    }
     s->setup_ms[PT_SETUP_PUSH] = ms_between(t1, std::chrono::steady_clock::now());
     *out = s;
     return PT_OK;
@@ -1310,9 +1320,27 @@ int pt_sim_run(pt_sim* s, int32_t* counts_out, int32_t* infected_out, int32_t* d
     if (read_state(s, &h)) return PT_EDEVICE;
     const int first = h.day, first_rows = h.days_run;
     HIP_TRY(hipEventRecord(s->run_begin, s->stream));
-    if (!h.done)
-        for (int d = first; d < s->params.max_days; ++d)
-            if (enqueue_day(s, d)) return PT_EDEVICE;
+    // Days go out in groups of kRunGroup.  After each group the loop's done
+    // flag is copied back; the host then waits for the group before (so one
+    // group stays queued and the GPU never idles) and stops enqueuing once
+    // that one had ended the loop.  Days enqueued after the end are no-ops
+    // (every kernel checks the flag), so at most two groups run idle.
+    constexpr int kRunGroup = 4;
+    if (!h.done) {
+        int d = first;
+        for (int g = 0; d < s->params.max_days; ++g) {
+            const int end = std::min(d + kRunGroup, s->params.max_days);
+            for (; d < end; ++d)
+                if (enqueue_day(s, d)) return PT_EDEVICE;
+            HIP_TRY(hipMemcpyAsync(&s->h_done[g & 1], &s->d_state->done, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                   s->stream));
+            HIP_TRY(hipEventRecord(s->group_ev[g & 1], s->stream));
+            if (g > 0) {
+                HIP_TRY(hipEventSynchronize(s->group_ev[(g - 1) & 1]));
+                if (s->h_done[(g - 1) & 1]) break;
+            }
+        }
+    }
     HIP_TRY(hipEventRecord(s->run_end, s->stream));
     HIP_TRY(hipEventSynchronize(s->run_end));
     float ms = 0.f;
@@ -1851,6 +1879,9 @@ void pt_sim_destroy(pt_sim* s) {
     if (s->run_end) (void)hipEventDestroy(s->run_end);
     for (auto e : s->setup_ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto e : s->group_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (s->h_done) (void)hipHostFree(s->h_done);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }

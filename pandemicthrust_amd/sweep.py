@@ -39,6 +39,10 @@ EXE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "bin", "pandemict
 # Device bytes per agent a batch needs, with headroom (agent arrays, the
 # 3N-pair errand tables and their sort scratch).
 BYTES_PER_AGENT = 200
+# include/pandemic/pt_batch.h: each run takes its population rounded up to
+# SLOT_TILE agent slots, and one batch holds at most MAX_SLOTS slots.
+SLOT_TILE = 8192
+MAX_SLOTS = 715827882
 
 
 def write_constants_file(experiment_name, max_days=100, rn_pandemic=1.8, rn_seasonal=1.3, initial_pandemic=30,
@@ -113,20 +117,43 @@ def read_constants(path):
     return Params.from_row(fields[1:10])
 
 
+def households(params):
+    """setup_scaleSimulation's household count (PandemicSim.cu:1133)."""
+    return int(np.floor(np.float64(np.float32(params.people_scaling)) * 1e6 + 0.5))
+
+
 def estimated_people(params):
     """About 2.4 agents per household (the reference's household tables)."""
-    return int(2.4 * round(params.people_scaling * 1e6)) + 16
+    return int(2.4 * households(params)) + 16
+
+
+def max_people(params):
+    """An upper bound on the population: the household tables' mean of 2.354
+    agents plus 8 standard deviations of a sum of H household sizes (each
+    1..5, standard deviation below 1.2), and never above 5 per household."""
+    H = households(params)
+    return int(min(5 * H, 2.354 * H + 8 * 1.2 * np.sqrt(H) + 16))
+
+
+def slots(params):
+    """Agent slots a run takes in a batch (pt_batch.h PT_BATCH_SLOT_TILE)."""
+    return -(-max_people(params) // SLOT_TILE) * SLOT_TILE
 
 
 def _chunks(runs, budget):
-    chunk, used = [], 0
+    """Consecutive runs grouped so that each group fits the memory budget
+    and the batch engine's slot limit (a run larger than either alone forms
+    a group of its own)."""
+    chunk, used, used_slots = [], 0, 0
     for r in runs:
         need = estimated_people(r["params"]) * BYTES_PER_AGENT
-        if chunk and used + need > budget:
+        sl = slots(r["params"])
+        if chunk and (used + need > budget or used_slots + sl > MAX_SLOTS):
             yield chunk
-            chunk, used = [], 0
+            chunk, used, used_slots = [], 0, 0
         chunk.append(r)
         used += need
+        used_slots += sl
     if chunk:
         yield chunk
 

@@ -25,6 +25,30 @@ ROW = ["40", "1.8", "1.3", "10", "10", "0.0425", "0.0425", "1", "0"]
 FILES = ["output_infected_stats.csv", "output_rn.csv", "infected_max.txt"]
 
 
+def _oracle_full(row, seed, iv=None):
+    """The same run on the CPU oracle."""
+    import oracle_ctypes as O
+    r = O.OracleSim(O.params_from_row(row), seed)
+    if iv is not None:
+        r.set_interventions(iv)
+    r.setup()
+    counts, inf = r.run()
+    return counts, inf, [r.array(k) for k in range(6)], r.generation_offsets()
+
+
+def _run_oracle_driver(run_dir, tmp):
+    """The oracle's driver on a copy of run_dir's constants.csv and seed.txt;
+    returns the directory holding its output files."""
+    import shutil
+    os.makedirs(tmp, exist_ok=True)
+    for name in ("constants.csv", "seed.txt"):
+        shutil.copy(os.path.join(run_dir, name), os.path.join(tmp, name))
+    r = subprocess.run([os.path.join(ROOT, "oracle", "build", "pt_oracle")], cwd=tmp, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.PIPE, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return tmp
+
+
 def _full(row, seed, iv=None):
     s = PandemicSim(Params.from_row(row), seed)
     if iv is not None:
@@ -40,6 +64,11 @@ def _full(row, seed, iv=None):
 @pytest.mark.parametrize("stop", [0, 1, 6, 12])
 def test_resume_matches_uninterrupted_run(stop):
     counts, inf, arrays, gens = _full(ROW, 3)
+    oc, oi, oa, og = _oracle_full(ROW, 3)
+    # the uninterrupted run is the oracle's; the resumed run must equal both
+    assert np.array_equal(counts, oc) and np.array_equal(inf, oi)
+    assert all(np.array_equal(a, b) for a, b in zip(arrays, oa))
+    assert np.array_equal(gens[0], og[0]) and np.array_equal(gens[1], og[1])
     a = PandemicSim(Params.from_row(ROW), 3)
     a.setup_sim()
     for _ in range(stop):
@@ -64,6 +93,9 @@ def test_resume_matches_uninterrupted_run(stop):
 def test_resume_with_interventions_and_after_the_end():
     iv = Interventions(school_close_start=3, school_close_days=10, workplace_reduction=0.3, vaccination=(0.1, 0.0))
     counts, inf, arrays, _ = _full(ROW, 5, iv)
+    oc, oi, oa, _ = _oracle_full(ROW, 5, iv)
+    assert np.array_equal(counts, oc) and np.array_equal(inf, oi)
+    assert all(np.array_equal(a, b) for a, b in zip(arrays, oa))
     a = PandemicSim(Params.from_row(ROW), 5)
     a.set_interventions(iv)
     a.setup_sim()
@@ -155,8 +187,11 @@ def test_host_program_checkpoint_resume(tmp_path):
     assert "checkpoint written" in out and not os.path.exists(os.path.join(part, "output_rn.csv"))
     out = _exe(part, "--resume", "state.bin")
     assert "resumed from state.bin at day 8" in out
+    o = _run_oracle_driver(full, str(tmp_path / "oracle"))
     for name in FILES:
-        assert open(os.path.join(full, name)).read() == open(os.path.join(part, name)).read(), name
+        text = open(os.path.join(o, name)).read()
+        assert open(os.path.join(full, name)).read() == text, name
+        assert open(os.path.join(part, name)).read() == text, name
 
 
 ROW_RE = re.compile(r"^(-?\d+), (\d+), ([A-Za-z_]+), (-?\d+), (-?\d+\.\d{6}), (-?\d+\.\d{6})$")
@@ -258,9 +293,12 @@ def test_sweep_batch_matches_single_runs(tmp_path):
     batch = sweep.run_sweep(paths, [1, 7], out_root=str(tmp_path / "batch"))
     single = sweep.run_sweep(paths, [1, 7], out_root=str(tmp_path / "single"), mode="process")
     assert len(batch) == len(single) == 8
-    for a, b in zip(batch, single):
+    for k, (a, b) in enumerate(zip(batch, single)):
+        o = _run_oracle_driver(a, str(tmp_path / "oracle" / str(k)))
         for name in FILES:
-            assert open(os.path.join(a, name)).read() == open(os.path.join(b, name)).read(), (a, name)
+            text = open(os.path.join(o, name)).read()
+            assert open(os.path.join(a, name)).read() == text, (a, name)
+            assert open(os.path.join(b, name)).read() == text, (b, name)
         for name in ("output_resource_log.csv", "profile_log.csv"):
             assert os.path.exists(os.path.join(a, name)), name
     # a tiny budget splits the sweep into several batches; results are the same

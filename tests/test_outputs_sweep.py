@@ -109,3 +109,43 @@ def test_host_program_rejects_bad_arguments():
     for args in (["--bogus"], ["--checkpoint", "x"], ["--checkpoint-day", "3"], ["a", "b"]):
         r = subprocess.run([EXE] + args, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=30)
         assert r.returncode == 2 and "usage" in r.stderr, args
+
+
+def _header_define(name):
+    import re
+    text = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "pandemic",
+                             "pt_batch.h")).read()
+    return int(re.search(r"#define\s+%s\s+(\d+)" % name, text).group(1))
+
+
+def test_chunks_fit_the_batch_slot_limit():
+    """The reference batcher's 32 experiments x 3 seeds on a 230 GB budget:
+    every chunk stays within the batch engine's agent-slot limit."""
+    from pandemicthrust_amd import Params
+    assert sweep.MAX_SLOTS == _header_define("PT_BATCH_MAX_SLOTS")
+    assert sweep.SLOT_TILE == _header_define("PT_BATCH_SLOT_TILE")
+    runs = []
+    for ps, ls in sweep.SCALES:
+        for ip in sweep.INITIAL_PANDEMIC:
+            for is_ in sweep.INITIAL_SEASONAL:
+                p = Params.from_row(["100", "1.8", "1.3", str(ip), str(is_), str(ps), str(ls), "1", "0"])
+                runs += [{"params": p, "seed": s} for s in (1, 2, 3)]
+    assert len(runs) == 96
+    chunks = list(sweep._chunks(runs, 230 * 10 ** 9))
+    assert sum(len(c) for c in chunks) == 96
+    for c in chunks:
+        assert sum(sweep.slots(r["params"]) for r in c) <= sweep.MAX_SLOTS or len(c) == 1
+        assert sum(sweep.estimated_people(r["params"]) * sweep.BYTES_PER_AGENT for r in c) <= 230 * 10 ** 9 \
+            or len(c) == 1
+
+
+def test_population_bound_holds():
+    """max_people bounds the population setup actually builds."""
+    import oracle_ctypes as O
+    from pandemicthrust_amd import Params
+    for ps, seed in (("0.0001", 1), ("0.001", 2), ("0.0425", 3), ("0.2", 4), ("1", 5)):
+        row = ["1", "1.8", "1.3", "1", "1", ps, ps, "1", "0"]
+        sim = O.OracleSim(O.params_from_row(row), seed)
+        sim.setup()
+        assert sim.n_people <= sweep.max_people(Params.from_row(row)), ps
+        assert sim.scalar(1) == sweep.households(Params.from_row(row))
