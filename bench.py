@@ -10,13 +10,26 @@ before each step; setup itself is input preparation and not timed).
 value = person-days per second over the timed steps
       = steps * N * days_simulated / wall seconds of those steps.
 
-roofline: the dominant kernel class by HIP-event time; `achieved` is its
-algorithmic bytes (each logical array read or written once, at its stored
-width, over the pairs the engine's per-day table log reports) divided by its
-launch time, both summed over the timed steps.  `traffic` is the class's
-measured HBM bytes per launch from the committed rocprofv3 summary
-(profiles/daily_loop_50M_100d.md: FETCH_SIZE + WRITE_SIZE runs of this same
-command, calibrated on the copy kernel).
+roofline: the dominant kernel class by HIP-event time (the radix downsweep
+of the full errand tables).  Its algorithmic bytes follow the loop model's
+table term: an ideal single-pass table build reads the E*N (key, id) pairs
+once (8 bytes each) and writes the sorted ids once (4 bytes), 12 bytes per
+pair, E = 2 on weekdays and 3 on weekends, counted on the days that build
+the full table (from the engine's per-day table log) and attributed to the
+downsweep launches; `achieved` = those bytes / the class's launch time over
+the timed steps.  `traffic` is the class's measured HBM bytes per launch
+from the committed rocprofv3 summary (profiles/daily_loop_50M_100d.md:
+FETCH_SIZE + WRITE_SIZE runs of this same command, calibrated on the copy
+kernel), and `traffic_over_algorithmic` their ratio.  `per_pass` keeps the
+engine's own byte model of the radix passes (every pass's reads and writes).
+loop_roofline: the loop model over the whole loop,
+B_wd = 43N + 89I + 3X + 8W + 4 and B_we = 62N + 62I + 3X + 40W + 4 per day.
+
+setup_ms: the population setup (households, workplaces, workplace table,
+status arrays) on the GPU; whole_program: bin/pandemicthrust run on the same
+constants.csv / seed.txt (process start to exit, setup, 100 days, output
+files), beside the reference's published whole-program time for 50M x 100
+days on a K20 (README.md:3, about 24 s).
 
 replicates: the 256-scenario set of pandemicthrust_amd/data/
 config2_scenarios.csv run to the end, setup included, once through the batch
@@ -32,8 +45,12 @@ re-simulation of flagged vectors) against simulating every vector, both
 plainly (one run at a time, timed on a sample) and on the batch engine.
 
 cpu_baseline: the CPU oracle (a sequential port of the reference semantics)
-on one host core, timed on a bounded sample of the same workload: setup plus
-the first CPU_DAYS days (setup excluded from the rate).
+on one host core, timed on a bounded sample of the same workload: the first
+CPU_DAYS days after setup, plus the epidemic's peak day (day PEAK_DAY,
+12.4M infected) continued from the GPU's state at that day boundary; the
+oracle's result of the peak day is compared with the GPU's (all six agent
+arrays).  The rate is person-days over those days / their seconds (setup
+excluded); the host's CPU model and core count are recorded.
 """
 import argparse
 import json
@@ -54,6 +71,8 @@ ROW = ["100", "1.8", "1.3", "10", "10", "21.24", "21.24", "1", "0"]
 SEED = 1
 HBM_PEAK_GBS = 8000.0
 CPU_DAYS = 4
+PEAK_DAY = 95
+K20_WHOLE_PROGRAM_S = 24.0   # README.md:3, 50M persons x 100 days on an NVIDIA K20
 CONSTANTS_HEADER = ("constants,max_days,reproduction_number_pandemic,reproduction_number_seasonal,"
                     "initial_infected_pandemic,initial_infected_seasonal,people_scaling_factor,"
                     "location_scaling_factor,percent_symptomatic,asymp_factor")
@@ -75,6 +94,9 @@ def algorithmic_bytes(N, W, lay, inf, counts, full, pairs):
 
     def add(k, b):
         per[k] = per.get(k, 0.0) + float(b)
+
+    # the loop model's table term: 12 bytes per pair on full-table days
+    add("table_model", 0.0)
 
     loop = 0.0
     days = len(inf)
@@ -101,6 +123,7 @@ def algorithmic_bytes(N, W, lay, inf, counts, full, pairs):
             loop += 43 * N + 89 * I + 3 * X + 8 * W + 4
         if d < len(full) and full[d]:
             n = 3 * N if weekend else 2 * N
+            add("table_model", 12 * n)
             # pairs kept: the weekday table leaves out the never-queried
             # hour-1 bucket (the first pass reads every key, writes the rest)
             k = int(pairs[d]) if d < len(pairs) and int(pairs[d]) > 0 else n
@@ -133,8 +156,69 @@ def measured_traffic(kernel):
     return int(m.group(1)) if m else None
 
 
-def cpu_baseline():
-    exe = os.path.join(ROOT, "oracle", "build", "pt_oracle")
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(peak):
+    """The oracle on one core: days 0..CPU_DAYS-1 after setup, then the peak
+    day from the GPU's state.  peak = (day, rand_offset, arrays before the
+    day, arrays after the GPU ran it)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ctypes as O
+    if not os.path.exists(O.LIB_PATH):
+        return None
+    t0 = time.perf_counter()
+    ref = O.OracleSim(O.params_from_row(ROW), SEED)
+    ref.setup()
+    setup_s = time.perf_counter() - t0
+    N = ref.n_people
+    t0 = time.perf_counter()
+    for _ in range(CPU_DAYS):
+        ref.step()
+    early_s = time.perf_counter() - t0
+    day, rand_offset, before, after = peak
+    ref.set_state(day, rand_offset, before)
+    t0 = time.perf_counter()
+    rc, _, inf = ref.step()
+    peak_s = time.perf_counter() - t0
+    identical = rc == 0 and all(np.array_equal(ref.array(k), after[k]) for k in range(6))
+    ref.close()
+    days = CPU_DAYS + 1
+    try:
+        cores_visible = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores_visible = os.cpu_count()
+    return {"value": round(N * days / (early_s + peak_s), 1), "unit": "person-days/s", "cores": 1, "kind": "port",
+            "sample": "oracle on one thread: days 0-%d of the 50M x 100d workload (%d agents; setup %.1f s untimed) "
+                      "in %.2f s, and day %d (the peak, %d infected) continued from the GPU's state in %.2f s"
+                      % (CPU_DAYS - 1, N, setup_s, early_s, day, inf, peak_s),
+            "early_days_per_s": round(N * CPU_DAYS / early_s, 1), "peak_day_s": round(peak_s, 3),
+            "peak_day_identical_to_gpu": bool(identical),
+            "cpu_model": cpu_model(), "host_cores": cores_visible}
+
+
+def peak_state(sim, day):
+    """The GPU run's state at the start of `day` and after it."""
+    sim.restore()
+    for _ in range(day):
+        sim.step()
+    sc = sim.scalars()
+    before = [sim.array(k) for k in range(6)]
+    sim.step()
+    after = [sim.array(k) for k in range(6)]
+    return int(sc[6]), int(sc[3]), before, after
+
+
+def whole_program():
+    """bin/pandemicthrust on the workload's constants.csv and seed.txt."""
+    exe = os.path.join(ROOT, "pandemicthrust_amd", "bin", "pandemicthrust")
     if not os.path.exists(exe):
         return None
     with tempfile.TemporaryDirectory() as d:
@@ -142,13 +226,13 @@ def cpu_baseline():
             f.write(CONSTANTS_HEADER + "\nconstants," + ",".join(ROW) + "\n")
         with open(os.path.join(d, "seed.txt"), "w") as f:
             f.write("%d\n" % SEED)
-        env = dict(os.environ, OMP_NUM_THREADS="1")
-        out = subprocess.run([exe, "--time", str(CPU_DAYS)], cwd=d, env=env, stdout=subprocess.PIPE, text=True,
-                             timeout=600, check=True).stdout
-    r = json.loads(out.strip().splitlines()[-1])
-    return {"value": round(r["person_days_per_s"], 1), "unit": "person-days/s", "cores": 1, "kind": "port",
-            "sample": "oracle setup (%.1f s, untimed) + first %d days of the 50M x 100d workload "
-                      "(%d agents, %.1f s loop)" % (r["setup_s"], r["days"], r["n_people"], r["loop_s"])}
+        t0 = time.perf_counter()
+        subprocess.run([exe], cwd=d, stdout=subprocess.DEVNULL, timeout=600, check=True)
+        t = time.perf_counter() - t0
+    return {"seconds": round(t, 3), "command": "pandemicthrust_amd/bin/pandemicthrust (constants.csv + seed.txt, "
+                                               "profile_log.csv on, as the reference's default build)",
+            "reference_k20_s": K20_WHOLE_PROGRAM_S, "reference_source": "README.md:3 (NVIDIA K20)",
+            "speedup_vs_reference": round(K20_WHOLE_PROGRAM_S / t, 1)}
 
 
 def replicate_bench(n_reps, plain=True, tune=True, solo_launch=None):
@@ -352,6 +436,7 @@ def main():
     t0 = time.perf_counter()
     N = sim.setup_sim()
     setup_s = time.perf_counter() - t0
+    setup_ms = sim.setup_times()
     sim.snapshot()
     lay = sim.layout()
     W = int(sim.scalars()[2])
@@ -410,8 +495,14 @@ def main():
     timed = {k: v for k, v in kt.items() if v[1] and k != "setup"}
     dom = max(timed, key=lambda k: timed[k][0])
     dom_ms, dom_launches = kt[dom]
-    dom_bytes = per_loop.get(dom, 0.0) * args.steps
+    # Algorithmic bytes of the dominant class: the loop model's table term
+    # when it is a table-build class, else the class's own byte model.
+    table_classes = ("sort_downsweep", "sort_upsweep")
+    model_bytes = per_loop["table_model"] if dom in table_classes else per_loop.get(dom, 0.0)
+    dom_bytes = model_bytes * args.steps
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    pass_bytes = per_loop.get(dom, 0.0) * args.steps
+    traffic = measured_traffic(dom)
     if args.kernels:
         for k, v in sorted(table.items(), key=lambda kv: -kv[1]["ms"]):
             print("%-18s %s" % (k, v), file=sys.stderr)
@@ -433,17 +524,28 @@ def main():
         "config": {"workload": "50M agents x 100 days: constants,100,1.8,1.3,10,10,21.24,21.24,1,0 seed 1",
                    "n_people": N, "days_per_step": days_per_step, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": os.path.relpath(PROFILE, ROOT),
                      "avg_launch_us": round(1000 * dom_ms / dom_launches, 2),
-                     "algorithmic_bytes_per_launch": round(dom_bytes / dom_launches, 1)},
+                     "algorithmic_bytes_per_launch": round(dom_bytes / dom_launches, 1),
+                     "algorithmic_model": "12 B per pair (read pairs 8 + write ids 4) x E*N pairs on full-table days"
+                     if dom in table_classes else "class byte model",
+                     "traffic_over_algorithmic": round(traffic / (dom_bytes / dom_launches), 2)
+                     if traffic and dom_bytes else None,
+                     "per_pass": {"bytes_per_launch": round(pass_bytes / dom_launches, 1),
+                                  "achieved": round(pass_bytes / (dom_ms * 1e-3) / 1e9, 1),
+                                  "frac": round(pass_bytes / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
         "loop_roofline": {"algorithmic_GB_per_loop": round(loop_bytes / 1e9, 3),
                           "achieved": round(loop_bytes * args.steps / loop_s / 1e9, 1),
                           "frac": round(loop_bytes * args.steps / loop_s / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels": table,
         "launch": {"settings": launch, "tuning_ms": tune_log},
+        "setup_ms": {"total_wall": round(1000 * setup_s, 2),
+                     **{k: round(v, 3) for k, v in setup_ms.items()}},
     }
+    peak = None if args.no_cpu_baseline else peak_state(sim, PEAK_DAY)
     sim.close()
+    line["whole_program"] = whole_program()
     if args.replicates > 0:
         line["replicates"] = replicate_bench(args.replicates, tune=not args.no_autotune, solo_launch=launch)
     if args.ensemble > 0:
@@ -451,7 +553,7 @@ def main():
     if args.surrogate > 0:
         line["surrogate"] = surrogate_bench(args.surrogate, 10000)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline()
+        line["cpu_baseline"] = cpu_baseline(peak)
     print(json.dumps(line), flush=True)
     return 0
 

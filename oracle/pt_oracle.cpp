@@ -946,6 +946,23 @@ int or_step(or_sim* s, int32_t counts16[16], int32_t* infected_count) {
     return 0;
 }
 
+int or_set_state(or_sim* s, int32_t day, uint64_t rand_offset, const uint8_t* st_p, const uint8_t* st_s,
+                 const uint8_t* day_p, const uint8_t* day_s, const uint8_t* gen_p, const uint8_t* gen_s) {
+    if (!s->setup_done || day < 0 || day > s->p.max_days) return -1;
+    const size_t n = (size_t)s->N;
+    std::memcpy(s->st_p.data(), st_p, n);
+    std::memcpy(s->st_s.data(), st_s, n);
+    std::memcpy(s->day_p.data(), day_p, n);
+    std::memcpy(s->day_s.data(), day_s, n);
+    std::memcpy(s->gen_p.data(), gen_p, n);
+    std::memcpy(s->gen_s.data(), gen_s, n);
+    s->current_day = day;
+    s->days_simulated = day;
+    s->rand_offset = rand_offset;
+    s->done = day >= s->p.max_days;
+    return 0;
+}
+
 int or_run(or_sim* s, int32_t* counts, int32_t* infected, int32_t cap_days) {
     int days = 0;
     while (days < cap_days) {

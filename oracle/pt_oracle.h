@@ -97,6 +97,11 @@ void or_set_interventions(or_sim* s, const or_interventions* iv);
  * Returns 0 if the day ran contacts, 1 if it was the terminal I==0 day or
  * max_days is reached (then counts are not written past max_days). */
 int or_step(or_sim* s, int32_t counts16[16], int32_t* infected_count);
+/* Continues a simulation from a saved day boundary (after or_setup): the six
+ * uint8 per-agent arrays, the day about to run and the running RNG offset
+ * (what a run reaches after `day` days).  Returns 0, or -1 before setup. */
+int or_set_state(or_sim* s, int32_t day, uint64_t rand_offset, const uint8_t* st_p, const uint8_t* st_s,
+                 const uint8_t* day_p, const uint8_t* day_s, const uint8_t* gen_p, const uint8_t* gen_s);
 /* Runs the remaining days; counts[days*16], infected[days]; returns days run. */
 int or_run(or_sim* s, int32_t* counts, int32_t* infected, int32_t cap_days);
 

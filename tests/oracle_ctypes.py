@@ -95,6 +95,7 @@ def lib(path=LIB_PATH):
         L.or_infection_probability.restype = C.c_float
         L.or_infection_probability.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.or_plant.restype = C.c_int
+        L.or_set_state.argtypes = [C.c_void_p, C.c_int32, C.c_uint64] + [C.c_void_p] * 6
         _libs[path] = L
     return L
 
@@ -240,6 +241,14 @@ class OracleSim:
         if n:
             self.L.or_get_trace(self.h, inf.ctypes.data, vic.ctypes.data, typ.ctypes.data, act.ctypes.data)
         return inf, vic, typ, act
+
+    def set_state(self, day, rand_offset, arrays):
+        """Continue from a day boundary: the six uint8 state arrays, the day
+        about to run and the RNG offset reached by then."""
+        a = [np.ascontiguousarray(x, np.uint8) for x in arrays]
+        rc = self.L.or_set_state(self.h, day, rand_offset, *[x.ctypes.data for x in a])
+        if rc != 0:
+            raise RuntimeError("or_set_state before setup")
 
     def trace_draws(self):
         """Per slot of the last contact day: draw bits (2 pandemic, 1 seasonal

@@ -187,3 +187,26 @@ def test_household_type_walk_is_the_inverse_cdf():
     assert set(got.tolist()) == set(range(len(cdf)))
     for i, c in enumerate(cdf[:-1]):
         assert O.fish_household_type(int(np.float64(c) * 2.0**32)) == i
+
+
+def test_oracle_continues_from_a_saved_day():
+    """or_set_state: a run continued from a copy of another run's state at a
+    day boundary ends exactly where the uninterrupted run ends (the CPU
+    baseline times peak days this way)."""
+    import oracle_ctypes as O
+    row = ["30", "1.8", "1.3", "10", "10", "0.0425", "0.0425", "1", "0"]
+    full = O.OracleSim(O.params_from_row(row), 3)
+    full.setup()
+    c_all, i_all = full.run()
+    a = O.OracleSim(O.params_from_row(row), 3)
+    a.setup()
+    for _ in range(12):
+        a.step()
+    state = [a.array(k) for k in range(6)]
+    b = O.OracleSim(O.params_from_row(row), 3)
+    b.setup()
+    b.set_state(a.scalar(6), a.scalar(3), state)
+    c_b, i_b = b.run()
+    assert np.array_equal(c_b, c_all[12:]) and np.array_equal(i_b, i_all[12:])
+    for k in range(6):
+        assert np.array_equal(b.array(k), full.array(k)), k
