@@ -57,6 +57,11 @@ int pt_mlp_gradients(pt_mlp* m, const int32_t* rows, float* grads_out, float* lo
 /* Predictions of every member for X[n][16]: out[(k*n + i)*720 + j]. */
 int pt_mlp_predict(pt_mlp* m, const float* X, int64_t n, float* out);
 
+/* Training steps are replayed from a HIP graph of one step captured per
+ * pt_mlp_train call (default), or enqueued kernel by kernel (on = 0); the
+ * results are identical. */
+int pt_mlp_use_graphs(pt_mlp* m, int32_t on);
+
 /* Milliseconds of the last pt_mlp_train (HIP events around the steps). */
 double pt_mlp_last_train_ms(pt_mlp* m);
 

@@ -36,7 +36,7 @@ EXPORTS = [
     "pt_sim_checkpoint_save", "pt_sim_checkpoint_load", "pt_device_memory", "pt_device_name",
     "pt_batch_use_graphs", "pt_sim_set_launch", "pt_sim_get_launch", "pt_batch_set_launch", "pt_batch_get_launch",
     "pt_sim_enable_trace", "pt_sim_trace_size", "pt_sim_trace", "pt_sim_trace_draws", "pt_location_table",
-    "pt_location_table_noalloc",
+    "pt_location_table_noalloc", "pt_mlp_use_graphs",
 ]
 
 # include/pandemic/pt_abi.h: largest input of pt_location_table_noalloc
@@ -159,6 +159,7 @@ def lib():
         L.pt_mlp_predict.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
         L.pt_mlp_last_train_ms.restype = C.c_double
         L.pt_mlp_last_train_ms.argtypes = [C.c_void_p]
+        L.pt_mlp_use_graphs.argtypes = [C.c_void_p, C.c_int32]
         L.pt_mlp_destroy.argtypes = [C.c_void_p]
         L.pt_sim_day_kernel_times.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
         L.pt_sim_setup_times.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]

@@ -185,8 +185,11 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm(GemmArgs g) {
 }
 
 // Batch gather: xb[b][i] = bf16(X[rows[b]][i]), yb[b][j] = Y[rows[b]][j].
-__global__ void k_gather(const float* __restrict__ X, const float* __restrict__ Y, const int32_t* __restrict__ rows,
-                         int B, bf16* __restrict__ xb, float* __restrict__ yb) {
+// step (may be null): [0] the step index within a training call, which
+// selects the batch's rows order + step[0] * B.
+__global__ void k_gather(const float* __restrict__ X, const float* __restrict__ Y, const int32_t* __restrict__ order,
+                         const int64_t* __restrict__ step, int B, bf16* __restrict__ xb, float* __restrict__ yb) {
+    const int32_t* rows = order + (step ? step[0] * B : 0);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t nx = (int64_t)B * PT_MLP_IN, ny = (int64_t)B * PT_MLP_OUT;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nx + ny; i += stride) {
@@ -242,13 +245,14 @@ __global__ __launch_bounds__(kLossThreads) void k_loss(const float* __restrict__
 }
 
 // Loss of member threadIdx.x from the block partials, in a fixed order.
+// step (may be null): [0] the step index, the loss's column.
 __global__ void k_loss_final(const float* __restrict__ part, int nblk, int members, int64_t n,
-                             float* __restrict__ loss, int64_t loss_z) {
+                             float* __restrict__ loss, int64_t loss_z, const int64_t* __restrict__ step) {
     const int z = threadIdx.x;
     if (z >= members) return;
     float t = 0.f;
     for (int b = 0; b < nblk; ++b) t += part[z * nblk + b];
-    loss[z * loss_z] = t / (float)n;
+    loss[z * loss_z + (step ? step[0] : 0)] = t / (float)n;
 }
 
 // Bias gradients: db[n] = sum over rows of dz[row][n], per member.  A block
@@ -275,8 +279,13 @@ __global__ __launch_bounds__(kColCols * kColGroups) void k_colsum(const float* _
 }
 
 // Adam on the FP32 master parameters, then their BF16 copies.
+// Adam with bias corrections 1 - beta^t for t = step[1] + 1 (the Adam
+// steps taken before this one, plus this one).
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
-                       float* __restrict__ m2, bf16* __restrict__ pb, int64_t n, float lr, float bc1, float bc2) {
+                       float* __restrict__ m2, bf16* __restrict__ pb, int64_t n, float lr,
+                       const int64_t* __restrict__ step) {
+    const double t = (double)(step[1] + 1);
+    const float bc1 = (float)(1.0 - pow(0.9, t)), bc2 = (float)(1.0 - pow(0.999, t));
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const float gi = g[i];
@@ -289,6 +298,12 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
         p[i] = w;
         pb[i] = (bf16)w;
     }
+}
+
+// Ends a training step: the next step's index and Adam count.
+__global__ void k_step_done(int64_t* step) {
+    step[0] += 1;
+    step[1] += 1;
 }
 
 }  // namespace
@@ -319,6 +334,11 @@ struct pt_mlp {
     int64_t xin_cap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0;
+    // Training steps: [0] step index within the current call, [1] Adam steps
+    // taken.  One step is captured as a HIP graph per training call and
+    // replayed (use_graphs), or enqueued kernel by kernel.
+    int64_t* d_step = nullptr;
+    bool use_graphs = true;
 };
 
 namespace {
@@ -370,12 +390,14 @@ void forward(pt_mlp* m, int rows) {
 }
 
 // Loss and gradients of every member for the batch in xb / yb.
-void backward(pt_mlp* m, float* loss, int64_t loss_z) {
+// step (may be null): the training step counters (the loss column).
+void backward(pt_mlp* m, float* loss, int64_t loss_z, const int64_t* step) {
     const int64_t P = m->L.count;
     const int B = m->batch;
     k_loss<<<dim3(kLossBlocks, m->members), kLossThreads, 0, m->stream>>>(m->pred, m->yb, B, m->dz32[0], m->dz16[0],
                                                                           dz_z(m), m->part);
-    k_loss_final<<<1, 64, 0, m->stream>>>(m->part, kLossBlocks, m->members, (int64_t)B * PT_MLP_OUT, loss, loss_z);
+    k_loss_final<<<1, 64, 0, m->stream>>>(m->part, kLossBlocks, m->members, (int64_t)B * PT_MLP_OUT, loss, loss_z,
+                                          step);
     int cur = 0;
     for (int l = PT_MLP_LAYERS - 1; l >= 0; --l) {
         const int nout = kDims[l + 1], nin = kDims[l];
@@ -471,6 +493,10 @@ int pt_mlp_create(int32_t members, int32_t batch, int32_t device, pt_mlp** out) 
     m->act[0] = m->xb;
     (void)hipEventCreate(&m->ev0);
     (void)hipEventCreate(&m->ev1);
+    if (dev_alloc(&m->d_step, 2)) {
+        pt_mlp_destroy(m);
+        return PT_ENOMEM;
+    }
     *out = m;
     return PT_OK;
 }
@@ -522,21 +548,46 @@ int pt_mlp_train(pt_mlp* m, const int32_t* order, int32_t steps, float lr, float
         m->loss_cap = (int64_t)steps * m->members;
     }
     const int64_t P = m->L.count * m->members;
+    // Every kernel of a step reads the step index and Adam count from
+    // d_step, so one captured step serves all of them.
+    const int64_t h_step[2] = {0, m->t};
+    HIP_TRY(hipMemcpyAsync(m->d_step, h_step, sizeof h_step, hipMemcpyHostToDevice, m->stream));
+    auto one_step = [&]() {
+        k_gather<<<grid_for((int64_t)B * (PT_MLP_IN + PT_MLP_OUT), 256, 4096), 256, 0, m->stream>>>(
+            m->X, m->Y, m->order, m->d_step, B, m->xb, m->yb);
+        forward(m, B);
+        backward(m, m->loss, steps, m->d_step);
+        k_adam<<<grid_for(P, 256, 8192), 256, 0, m->stream>>>(m->params, m->grads, m->m1, m->m2, m->wb, P, lr,
+                                                               m->d_step);
+        k_step_done<<<1, 1, 0, m->stream>>>(m->d_step);
+    };
+    hipGraphExec_t exec = nullptr;
+    if (m->use_graphs) {
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+        one_step();
+        HIP_TRY(hipStreamEndCapture(m->stream, &graph));
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(PT_EDEVICE, "training graph: %s", hipGetErrorString(e));
+    }
     HIP_TRY(hipEventRecord(m->ev0, m->stream));
     for (int s = 0; s < steps; ++s) {
-        k_gather<<<grid_for((int64_t)B * (PT_MLP_IN + PT_MLP_OUT), 256, 4096), 256, 0, m->stream>>>(
-            m->X, m->Y, m->order + (int64_t)s * B, B, m->xb, m->yb);
-        forward(m, B);
-        backward(m, m->loss + s, steps);
-        m->t += 1;
-        const float bc1 = (float)(1.0 - std::pow(0.9, (double)m->t));
-        const float bc2 = (float)(1.0 - std::pow(0.999, (double)m->t));
-        k_adam<<<grid_for(P, 256, 8192), 256, 0, m->stream>>>(m->params, m->grads, m->m1, m->m2, m->wb, P, lr, bc1,
-                                                               bc2);
+        if (exec) {
+            const hipError_t e = hipGraphLaunch(exec, m->stream);
+            if (e != hipSuccess) {
+                (void)hipGraphExecDestroy(exec);
+                return fail(PT_EDEVICE, "training graph launch: %s", hipGetErrorString(e));
+            }
+        } else {
+            one_step();
+        }
     }
+    m->t += steps;
     HIP_TRY(hipEventRecord(m->ev1, m->stream));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventSynchronize(m->ev1));
+    if (exec) (void)hipGraphExecDestroy(exec);
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
     m->last_ms = ms;
@@ -561,10 +612,10 @@ int pt_mlp_gradients(pt_mlp* m, const int32_t* rows, float* grads_out, float* lo
         if (dev_alloc(&m->loss, m->members)) return PT_ENOMEM;
         m->loss_cap = m->members;
     }
-    k_gather<<<grid_for((int64_t)B * (PT_MLP_IN + PT_MLP_OUT), 256, 4096), 256, 0, m->stream>>>(m->X, m->Y, m->order,
-                                                                                             B, m->xb, m->yb);
+    k_gather<<<grid_for((int64_t)B * (PT_MLP_IN + PT_MLP_OUT), 256, 4096), 256, 0, m->stream>>>(
+        m->X, m->Y, m->order, (const int64_t*)nullptr, B, m->xb, m->yb);
     forward(m, B);
-    backward(m, m->loss, 1);
+    backward(m, m->loss, 1, nullptr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(m->stream));
     HIP_TRY(hipMemcpy(grads_out, m->grads, sizeof(float) * m->L.count * m->members, hipMemcpyDeviceToHost));
@@ -599,6 +650,12 @@ int pt_mlp_predict(pt_mlp* m, const float* X, int64_t n, float* out) {
     return PT_OK;
 }
 
+int pt_mlp_use_graphs(pt_mlp* m, int32_t on) {
+    if (!m) return fail(PT_EINVAL, "null handle");
+    m->use_graphs = on != 0;
+    return PT_OK;
+}
+
 double pt_mlp_last_train_ms(pt_mlp* m) { return m ? m->last_ms : -1.0; }
 
 void pt_mlp_destroy(pt_mlp* m) {
@@ -627,6 +684,7 @@ void pt_mlp_destroy(pt_mlp* m) {
     dev_free(m->xin);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
     if (m->ev1) (void)hipEventDestroy(m->ev1);
+    dev_free(m->d_step);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
 }

@@ -15,6 +15,14 @@ R in [1.0, 2.5]).  Scenario vectors are drawn from one std::mt19937_64 stream
 (scenarios.MT19937_64), u = (x >> 11) 2^-53 per dimension in order.
 
     python -m pandemicthrust_amd.ensemble --scenarios 64 --replicates 8 --agents 100000 --days 120 --out ens.npz
+
+The surrogate's training set (4,096 scenarios x 64 replicates at 1,000,000
+agents x 120 days, vectors from seed C3_SEED) is generated in parts and
+merged into pandemicthrust_amd/data/surrogate_ensemble.npz:
+
+    python -m pandemicthrust_amd.ensemble --scenarios 4096 --replicates 64 --agents 1000000 --days 120 \
+        --seed 20261204 --range 0:2048 --out part0.npz
+    python -m pandemicthrust_amd.ensemble --merge part0.npz part1.npz --out surrogate_ensemble.npz
 """
 import argparse
 import math
@@ -64,11 +72,17 @@ def neutral_vector(p_pandemic=0.018, p_seasonal=0.013, init=(10, 10)):
     return np.array([p_pandemic, p_seasonal, init[0], init[1], 0, 0, 10, 10, 0, 0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0])
 
 
-def scenario_inputs(vec, n_agents, days):
-    """(Params, Interventions) of one scenario vector at a target population."""
+def scenario_inputs(vec, n_agents, days, seeds_per_million=False):
+    """(Params, Interventions) of one scenario vector at a target population.
+    With seeds_per_million the initial-infected dimensions count per million
+    agents (the vector's value times n_agents / 1e6, rounded), which keeps
+    the per-capita start of a scenario learnt at 1M agents at any size."""
     v = [float(x) for x in vec]
     scale = "%.6f" % (n_agents / DEFAULT_AGENTS_PER_SCALE)
-    row = [str(int(days)), "%.4f" % (100.0 * v[0]), "%.4f" % (100.0 * v[1]), str(int(v[2])), str(int(v[3])), scale,
+    seeds = [int(v[2]), int(v[3])]
+    if seeds_per_million:
+        seeds = [max(1, int(round(x * n_agents / 1e6))) for x in seeds]
+    row = [str(int(days)), "%.4f" % (100.0 * v[0]), "%.4f" % (100.0 * v[1]), str(seeds[0]), str(seeds[1]), scale,
            scale, "%.4f" % v[14], "%.4f" % v[15]]
     iv = Interventions(latent=(int(v[4]), int(v[5])), infectious=(int(v[6]), int(v[7])),
                        school_close_start=int(v[8]), school_close_days=int(v[9]), workplace_reduction=v[10],
@@ -95,7 +109,7 @@ def summarise(inc):
 
 
 def run_ensemble(vectors, replicates, n_agents, days, seed0=1, max_agents_per_batch=200_000_000, device=0,
-                 return_raw=False):
+                 return_raw=False, seeds_per_million=False):
     """Runs every scenario with `replicates` seeds (seed0 + scenario * replicates
     + r) in as few batches as fit max_agents_per_batch.  Returns Y
     (scenarios x 2 x days x 3) and timing; with return_raw also the raw
@@ -103,6 +117,8 @@ def run_ensemble(vectors, replicates, n_agents, days, seed0=1, max_agents_per_ba
     vectors = np.asarray(vectors)
     n = len(vectors)
     per_batch = max(1, int(max_agents_per_batch // (n_agents * replicates)))
+    # replicates of one scenario may be split over batches when they do not fit one
+    rep_split = max(1, -(-n_agents * replicates // max_agents_per_batch))
     Y = np.zeros((n, 2, days, 3))
     raw = []
     person_days = 0
@@ -111,16 +127,19 @@ def run_ensemble(vectors, replicates, n_agents, days, seed0=1, max_agents_per_ba
         chunk = range(s0, min(n, s0 + per_batch))
         params, ivs, seeds = [], [], []
         for i in chunk:
-            p, iv = scenario_inputs(vectors[i], n_agents, days)
+            p, iv = scenario_inputs(vectors[i], n_agents, days, seeds_per_million)
             for r in range(replicates):
                 params.append(p)
                 ivs.append(iv)
                 seeds.append(seed0 + i * replicates + r)
-        b = PandemicBatch(params, seeds, device)
-        b.set_interventions(ivs)
-        n_people = b.setup()
-        res = b.run()
-        b.close()
+        res, n_people = [], []
+        step = -(-len(params) // rep_split)
+        for b0 in range(0, len(params), step):
+            b = PandemicBatch(params[b0:b0 + step], seeds[b0:b0 + step], device)
+            b.set_interventions(ivs[b0:b0 + step])
+            n_people.extend(int(x) for x in b.setup())
+            res.extend(b.run())
+            b.close()
         for j, i in enumerate(chunk):
             runs = res[j * replicates:(j + 1) * replicates]
             inc = np.stack([incidence(c, days) for c, _ in runs])
@@ -135,6 +154,39 @@ def run_ensemble(vectors, replicates, n_agents, days, seed0=1, max_agents_per_ba
     return (Y, info, raw) if return_raw else (Y, info)
 
 
+C3_SEED = 20261204          # scenario vectors of the surrogate's training set
+C3_SEED0 = 1                # replicate seeds: C3_SEED0 + scenario * replicates + r
+
+
+def save_part(path, X, Y, first, info, replicates, agents, days, seed):
+    """A generated part: scenario rows [first, first + len(X)) of the vector
+    stream `seed`, targets in float16 (log-incidence values up to ~15)."""
+    np.savez_compressed(path, X=X, Y=Y.astype(np.float16), first=np.int64(first), replicates=np.int64(replicates),
+                        agents=np.int64(agents), days=np.int64(days), seed=np.int64(seed),
+                        seconds=np.float64(info["seconds"]), person_days=np.int64(info["person_days"]))
+
+
+def merge_parts(paths, out):
+    """Concatenates parts in row order (they must tile [0, n) of one stream)."""
+    parts = sorted((np.load(p) for p in paths), key=lambda z: int(z["first"]))
+    nxt = 0
+    for z in parts:
+        assert int(z["first"]) == nxt, "parts do not tile the scenario range"
+        nxt += len(z["X"])
+    meta = {k: int(parts[0][k]) for k in ("replicates", "agents", "days", "seed")}
+    np.savez_compressed(out, X=np.concatenate([z["X"] for z in parts]), Y=np.concatenate([z["Y"] for z in parts]),
+                        seconds=np.float64(sum(float(z["seconds"]) for z in parts)),
+                        person_days=np.int64(sum(int(z["person_days"]) for z in parts)), **meta)
+
+
+def load_training_set(path):
+    """(X float64, Y float32 n x 2 x days x 3, metadata) of a merged set."""
+    z = np.load(path)
+    meta = {k: (float(z[k]) if k == "seconds" else int(z[k])) for k in
+            ("replicates", "agents", "days", "seed", "seconds", "person_days")}
+    return z["X"], z["Y"].astype(np.float32), meta
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--scenarios", type=int, default=64)
@@ -142,12 +194,31 @@ def main():
     ap.add_argument("--agents", type=int, default=100000)
     ap.add_argument("--days", type=int, default=120)
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--seed0", type=int, default=1, help="first replicate seed")
+    ap.add_argument("--range", default=None, help="first:stop scenario rows to generate (default: all)")
+    ap.add_argument("--chunk", type=int, default=64, help="scenarios per progress line")
+    ap.add_argument("--merge", nargs="+", default=None, help="merge generated parts instead")
     ap.add_argument("--out", default="ensemble.npz")
     a = ap.parse_args()
+    if a.merge:
+        merge_parts(a.merge, a.out)
+        print("merged %d parts into %s" % (len(a.merge), a.out))
+        return
     X = sample_scenarios(a.scenarios, a.seed)
-    Y, info = run_ensemble(X, a.replicates, a.agents, a.days)
-    np.savez(a.out, X=X, Y=Y)
-    print(info)
+    first, stop = (0, a.scenarios) if a.range is None else (int(x) for x in a.range.split(":"))
+    Ys, secs, pdays = [], 0.0, 0
+    for c0 in range(first, stop, a.chunk):
+        c1 = min(stop, c0 + a.chunk)
+        # replicate seeds follow the scenario's row in the whole stream
+        Y, info = run_ensemble(X[c0:c1], a.replicates, a.agents, a.days, seed0=a.seed0 + c0 * a.replicates)
+        Ys.append(Y)
+        secs += info["seconds"]
+        pdays += info["person_days"]
+        print("rows %d-%d: %d runs in %.1f s (%.1f runs/s)" % (c0, c1, info["runs"], info["seconds"],
+                                                                 info["runs_per_s"]), flush=True)
+    save_part(a.out, X[first:stop], np.concatenate(Ys), first, {"seconds": secs, "person_days": pdays}, a.replicates,
+              a.agents, a.days, a.seed)
+    print("%d scenarios in %.1f s -> %s" % (stop - first, secs, a.out), flush=True)
 
 
 if __name__ == "__main__":

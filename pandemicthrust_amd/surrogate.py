@@ -104,6 +104,11 @@ class Surrogate:
         _check(lib().pt_mlp_train(self.h, order.ctypes.data, steps, C.c_float(lr), loss.ctypes.data))
         return loss
 
+    def use_graphs(self, on=True):
+        """Training steps replayed from a HIP graph (default) or enqueued
+        kernel by kernel; results are identical."""
+        _check(lib().pt_mlp_use_graphs(self.h, 1 if on else 0))
+
     def last_train_ms(self):
         return float(lib().pt_mlp_last_train_ms(self.h))
 
@@ -123,23 +128,92 @@ class Surrogate:
         return out
 
 
-def screen(surrogate, X, train_X, disagreement=0.5, top=0.01):
-    """Surrogate screening of raw scenario vectors X (n x 16): the ensemble
-    mean prediction, its spread, and which rows go back to the simulator —
-    members disagree (mean over outputs of the ensemble standard deviation
-    of the log-incidence mean above `disagreement`), the row lies outside the
-    box of the training inputs, or it is in the top `top` share by predicted
-    pandemic attack (sum of predicted incidence)."""
-    Xn = normalize(X)
+def curve_stats(logmean, scale=1.0):
+    """Peak day, peak height and attack (total incidence) of incidence
+    curves given as log(1 + incidence) means (... x days), with incidence
+    multiplied by `scale` (a population ratio)."""
+    inc = np.expm1(np.clip(np.asarray(logmean, np.float64), 0.0, None)) * scale
+    return inc.argmax(axis=-1), inc.max(axis=-1), inc.sum(axis=-1)
+
+
+def curve_errors(pred_logmean, sim_logmean, pred_scale=1.0):
+    """Relative errors of the surrogate's curves against the simulator's
+    ensemble means (scenarios x days, log(1 + incidence)): peak day, peak
+    height and attack, each |pred - sim| / sim (sim at least 1)."""
+    pd, ph, pa = curve_stats(pred_logmean, pred_scale)
+    sd, sh, sa = curve_stats(sim_logmean)
+    return {"peak_day": np.abs(pd - sd) / np.maximum(sd, 1), "peak_height": np.abs(ph - sh) / np.maximum(sh, 1.0),
+            "attack": np.abs(pa - sa) / np.maximum(sa, 1.0), "peak_day_abs": np.abs(pd - sd).astype(np.float64)}
+
+
+def nearest_distance(Xn, train_n, chunk=4096):
+    """Euclidean distance from each scaled row of Xn to its nearest training
+    row (both in [0, 1]^16)."""
+    T = np.asarray(train_n, np.float64)
+    t2 = (T * T).sum(axis=1)
+    out = np.empty(len(Xn))
+    for i in range(0, len(Xn), chunk):
+        x = np.asarray(Xn[i:i + chunk], np.float64)
+        d2 = (x * x).sum(axis=1)[:, None] + t2[None, :] - 2.0 * x @ T.T
+        out[i:i + chunk] = np.sqrt(np.maximum(d2.min(axis=1), 0.0))
+    return out
+
+
+def in_hull(x, train_n):
+    """Whether the scaled point x lies in the convex hull of the scaled
+    training rows: a feasible lambda >= 0 with sum 1 and train^T lambda = x
+    (a linear program, scipy's HiGHS)."""
+    from scipy.optimize import linprog
+    T = np.asarray(train_n, np.float64)
+    A = np.vstack([T.T, np.ones(len(T))])
+    b = np.concatenate([np.asarray(x, np.float64), [1.0]])
+    r = linprog(np.zeros(len(T)), A_eq=A, b_eq=b, bounds=(0, None), method="highs")
+    return r.status == 0
+
+
+def member_spread(P, days=120):
+    """Mean over days and strains of the members' standard deviation of the
+    predicted log-incidence mean (P: members x n x 720)."""
+    return P.std(axis=0).reshape(P.shape[1], 2, days, 3)[..., 0].mean(axis=(1, 2))
+
+
+def calibrate(surrogate, heldout_X, train_X, quantile=0.99):
+    """Re-simulation thresholds from held-out in-distribution scenarios (raw
+    vectors): the `quantile` of the members' spread and of the distance to the
+    nearest training scenario."""
+    Xn, Tn = normalize(heldout_X), normalize(train_X)
+    spread = member_spread(surrogate.predict(Xn))
+    return {"disagreement": float(np.quantile(spread, quantile)),
+            "distance": float(np.quantile(nearest_distance(Xn, Tn), quantile)), "quantile": quantile}
+
+
+def screen(surrogate, X, train_X, thresholds, top=0.01, days=120):
+    """Surrogate screening of raw scenario vectors X (n x 16; train_X: the
+    raw training vectors, both scaled here).  Returns the ensemble mean
+    prediction, its spread, and which rows go back to the simulator:
+    members disagree (spread above thresholds["disagreement"]), the row is
+    out of distribution (outside the training ranges in some dimension, or
+    farther from its nearest training scenario than
+    thresholds["distance"]), or it is in the top `top` share by predicted
+    pandemic attack (sum of predicted incidence over the first `days` days).
+
+    Out of distribution is not tested against the convex hull of the
+    training set: in 16 dimensions almost no new point lies inside the hull
+    of a few thousand uniform samples (in_hull), so that test would send
+    every row back; the nearest-training-scenario distance, calibrated on
+    held-out scenarios, measures the same thing usefully."""
+    Xn, Tn = normalize(X), normalize(train_X)
     P = surrogate.predict(Xn)
     mean = P.mean(axis=0)
-    spread = P.std(axis=0).reshape(len(X), 2, 120, 3)[..., 0].mean(axis=(1, 2))
-    lo, hi = np.min(train_X, axis=0), np.max(train_X, axis=0)
-    outside = ((Xn < lo) | (Xn > hi)).any(axis=1)
-    attack = np.expm1(np.clip(mean.reshape(len(X), 2, 120, 3)[:, 0, :, 0], 0, None)).sum(axis=1)
+    spread = member_spread(P)
+    lo, hi = Tn.min(axis=0), Tn.max(axis=0)
+    dist = nearest_distance(Xn, Tn)
+    outside = ((Xn < lo) | (Xn > hi)).any(axis=1) | (dist > thresholds["distance"])
+    disagree = spread > thresholds["disagreement"]
+    attack = np.expm1(np.clip(mean.reshape(len(X), 2, 120, 3)[:, 0, :days, 0], 0, None)).sum(axis=1)
     k = max(1, int(math.ceil(top * len(X))))
     top_rows = np.zeros(len(X), bool)
     top_rows[np.argsort(-attack)[:k]] = True
-    resim = (spread > disagreement) | outside | top_rows
-    return {"mean": mean, "spread": spread, "outside": outside, "top": top_rows, "resimulate": resim,
-            "attack": attack}
+    resim = disagree | outside | top_rows
+    return {"mean": mean, "spread": spread, "distance": dist, "outside": outside, "disagree": disagree,
+            "top": top_rows, "resimulate": resim, "attack": attack}

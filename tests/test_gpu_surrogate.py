@@ -99,3 +99,29 @@ def test_training_reduces_the_loss():
     first, last = loss[:10].mean(axis=0), loss[-10:].mean(axis=0)
     assert (last < 0.5 * first).all(), (first, last)
     s.close()
+
+
+def test_graph_replayed_training_equals_eager_training():
+    """Training steps replayed from the captured HIP graph give the same
+    parameters and losses as the same steps enqueued kernel by kernel: one
+    call of 7 steps (7 replays), then 3 calls of 2 steps with host work and
+    synchronisation between them, each compared with the eager run."""
+    from pandemicthrust_amd.surrogate import Surrogate
+    X, Y = _data(1024, seed=5)
+    runs = []
+    for graphs in (True, False):
+        s = Surrogate(members=3, batch=256, seed=4)
+        s.use_graphs(graphs)
+        s.set_data(X, Y)
+        losses = [s.train(7, lr=1e-3, seed=1)]
+        params = [s.params()]
+        for k in range(3):
+            losses.append(s.train(2, lr=3e-4, seed=10 + k))
+            params.append(s.params())
+            np.asarray(params[-1]).sum()    # host work between the calls
+        s.close()
+        runs.append((losses, params))
+    for a, b in zip(runs[0][0], runs[1][0]):
+        assert np.array_equal(a, b)
+    for a, b in zip(runs[0][1], runs[1][1]):
+        assert np.array_equal(a, b)
