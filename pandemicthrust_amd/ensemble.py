@@ -72,11 +72,30 @@ def neutral_vector(p_pandemic=0.018, p_seasonal=0.013, init=(10, 10)):
     return np.array([p_pandemic, p_seasonal, init[0], init[1], 0, 0, 10, 10, 0, 0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0])
 
 
-def scenario_inputs(vec, n_agents, days, seeds_per_million=False):
-    """(Params, Interventions) of one scenario vector at a target population.
-    With seeds_per_million the initial-infected dimensions count per million
-    agents (the vector's value times n_agents / 1e6, rounded), which keeps
-    the per-capita start of a scenario learnt at 1M agents at any size."""
+# Dimension pairs (pandemic, seasonal) of the per-strain parameters.  The
+# model treats the two strains alike: each has its own transmission factor,
+# initial infected, latent and infectious days and vaccination share, and
+# both use the same contacts, the same infection-profile function and
+# identically distributed random words (PandemicSim.cu:3106-3233, :708-758),
+# so a scenario with these pairs swapped has the same distribution of
+# outcomes with the strains swapped.
+STRAIN_PAIRS = [(0, 1), (2, 3), (4, 5), (6, 7), (12, 13)]
+
+
+def swap_strains(X):
+    """Scenario vectors with every per-strain dimension pair swapped."""
+    X = np.array(X, copy=True)
+    for a, b in STRAIN_PAIRS:
+        X[..., [a, b]] = X[..., [b, a]]
+    return X
+
+
+def scenario_row(vec, n_agents, days, seeds_per_million=False):
+    """(constants row, Interventions) of one scenario vector at a target
+    population.  With seeds_per_million the initial-infected dimensions
+    count per million agents (the vector's value times n_agents / 1e6,
+    rounded), which keeps the per-capita start of a scenario learnt at 1M
+    agents at any size."""
     v = [float(x) for x in vec]
     scale = "%.6f" % (n_agents / DEFAULT_AGENTS_PER_SCALE)
     seeds = [int(v[2]), int(v[3])]
@@ -87,6 +106,12 @@ def scenario_inputs(vec, n_agents, days, seeds_per_million=False):
     iv = Interventions(latent=(int(v[4]), int(v[5])), infectious=(int(v[6]), int(v[7])),
                        school_close_start=int(v[8]), school_close_days=int(v[9]), workplace_reduction=v[10],
                        errand_reduction=v[11], vaccination=(v[12], v[13]))
+    return row, iv
+
+
+def scenario_inputs(vec, n_agents, days, seeds_per_million=False):
+    """(Params, Interventions) of one scenario vector (scenario_row)."""
+    row, iv = scenario_row(vec, n_agents, days, seeds_per_million)
     return Params.from_row(row), iv
 
 

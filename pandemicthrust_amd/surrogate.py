@@ -12,7 +12,7 @@ import math
 import numpy as np
 
 from .api import _check, lib
-from .ensemble import SCENARIO_DIMS
+from .ensemble import SCENARIO_DIMS, swap_strains
 
 IN, HIDDEN, OUT = 16, 512, 720
 DIMS = [IN, HIDDEN, HIDDEN, HIDDEN, HIDDEN, OUT]
@@ -30,6 +30,15 @@ def targets(Y):
     Y = np.asarray(Y)
     assert Y.shape[1:] == (2, 120, 3), Y.shape
     return Y.reshape(len(Y), OUT).astype(np.float32)
+
+
+def augment(X, Y):
+    """Training rows with their strain-swapped twins appended: the scenario
+    with its per-strain dimensions swapped (ensemble.swap_strains) has the
+    strains' outcomes swapped, so its targets are Y with the strain axis
+    reversed.  X: n x 16 raw vectors, Y: n x 2 x days x 3."""
+    X, Y = np.asarray(X), np.asarray(Y)
+    return np.concatenate([X, swap_strains(X)]), np.concatenate([Y, Y[:, ::-1]])
 
 
 def glorot(seed, members):
