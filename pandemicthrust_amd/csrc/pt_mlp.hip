@@ -255,27 +255,42 @@ __global__ void k_loss_final(const float* __restrict__ part, int nblk, int membe
     loss[z * loss_z + (step ? step[0] : 0)] = t / (float)n;
 }
 
-// Bias gradients: db[n] = sum over rows of dz[row][n], per member.  A block
-// of 64 columns x 16 row groups; the 16 partial sums are added in a fixed
-// order, so the result does not depend on scheduling.
-constexpr int kColCols = 64, kColGroups = 16;
-__global__ __launch_bounds__(kColCols * kColGroups) void k_colsum(const float* __restrict__ dz, int M, int N,
-                                                                 int64_t dz_z, float* __restrict__ db, int64_t db_z) {
-    const int z = blockIdx.y;
+// Bias gradients: db[n] = sum over rows of dz[row][n], per member, in two
+// launches.  k_colsum_part: a block sums a chunk of kColRows rows for 64
+// columns (4 row groups, added in a fixed order) into part[z][chunk][n];
+// k_colsum_final adds the chunks in order.  The grid covers columns x row
+// chunks x members, so the sums spread over the chip, and the result does
+// not depend on scheduling.
+constexpr int kColCols = 64, kColGroups = 4, kColRows = 128;
+__global__ __launch_bounds__(kColCols * kColGroups) void k_colsum_part(const float* __restrict__ dz, int M, int N,
+                                                                      int64_t dz_z, float* __restrict__ part) {
+    const int z = blockIdx.z, chunk = blockIdx.y, R = gridDim.y;
     const int c = threadIdx.x % kColCols, grp = threadIdx.x / kColCols;
     const int n = blockIdx.x * kColCols + c;
     const float* d = dz + z * dz_z;
+    const int m1 = min(M, (chunk + 1) * kColRows);
     float s = 0.f;
     if (n < N)
-        for (int m = grp; m < M; m += kColGroups) s += d[(int64_t)m * N + n];
+        for (int m = chunk * kColRows + grp; m < m1; m += kColGroups) s += d[(int64_t)m * N + n];
     __shared__ float s_p[kColGroups][kColCols];
     s_p[grp][c] = s;
     __syncthreads();
     if (grp == 0 && n < N) {
         float t = 0.f;
+#pragma unroll
         for (int q = 0; q < kColGroups; ++q) t += s_p[q][c];
-        db[z * db_z + n] = t;
+        part[((int64_t)z * R + chunk) * N + n] = t;
     }
+}
+
+__global__ void k_colsum_final(const float* __restrict__ part, int R, int N, int members, float* __restrict__ db,
+                               int64_t db_z) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= members * N) return;
+    const int z = i / N, n = i % N;
+    float t = 0.f;
+    for (int r = 0; r < R; ++r) t += part[((int64_t)z * R + r) * N + n];
+    db[z * db_z + n] = t;
 }
 
 // Adam on the FP32 master parameters, then their BF16 copies.
@@ -328,6 +343,7 @@ struct pt_mlp {
     float* dz32[2] = {};
     bf16* dz16[2] = {};
     float* part = nullptr;
+    float* colpart = nullptr;  // bias-gradient partial sums [members][row chunks][kMaxWidth]
     float* loss = nullptr;
     int64_t loss_cap = 0;
     float* xin = nullptr;  // prediction inputs
@@ -417,8 +433,11 @@ void backward(pt_mlp* m, float* loss, int64_t loss_z, const int64_t* step) {
         g.ldc = nout;
         gemm<true, false, EPI_F32>(m, g);
         // db_l
-        k_colsum<<<dim3((nout + kColCols - 1) / kColCols, m->members), kColCols * kColGroups, 0, m->stream>>>(
-            m->dz32[cur], B, nout, dz_z(m), m->grads + m->L.b[l], P);
+        const int R = (B + kColRows - 1) / kColRows;
+        k_colsum_part<<<dim3((nout + kColCols - 1) / kColCols, R, m->members), kColCols * kColGroups, 0,
+                        m->stream>>>(m->dz32[cur], B, nout, dz_z(m), m->colpart);
+        k_colsum_final<<<(m->members * nout + 255) / 256, 256, 0, m->stream>>>(m->colpart, R, nout, m->members,
+                                                                              m->grads + m->L.b[l], P);
         if (l == 0) break;
         // dz_{l-1} = (dz_l W_l^T) * gelu'(pre_{l-1})
         GemmArgs h = args();
@@ -480,7 +499,8 @@ int pt_mlp_create(int32_t members, int32_t batch, int32_t device, pt_mlp** out) 
                dev_alloc(&m->wb, P) || dev_alloc(&m->xb, (int64_t)batch * PT_MLP_IN) ||
                dev_alloc(&m->yb, (int64_t)batch * PT_MLP_OUT) ||
                dev_alloc(&m->pred, (int64_t)members * batch * PT_MLP_OUT) ||
-               dev_alloc(&m->part, (int64_t)members * kLossBlocks);
+               dev_alloc(&m->part, (int64_t)members * kLossBlocks) ||
+               dev_alloc(&m->colpart, (int64_t)members * ((batch + kColRows - 1) / kColRows) * kMaxWidth);
     for (int l = 1; l < PT_MLP_LAYERS && !bad; ++l) bad = dev_alloc(&m->act[l], members * act_z(m)) != 0;
     for (int l = 0; l < PT_MLP_LAYERS - 1 && !bad; ++l) bad = dev_alloc(&m->pre[l], members * act_z(m)) != 0;
     for (int i = 0; i < 2 && !bad; ++i)
@@ -680,6 +700,7 @@ void pt_mlp_destroy(pt_mlp* m) {
     }
     dev_free(m->pred);
     dev_free(m->part);
+    dev_free(m->colpart);
     dev_free(m->loss);
     dev_free(m->xin);
     if (m->ev0) (void)hipEventDestroy(m->ev0);

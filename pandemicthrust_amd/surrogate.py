@@ -6,8 +6,11 @@ Inputs are scenario vectors (ensemble.SCENARIO_DIMS) scaled to [0, 1] per
 dimension; outputs are ensemble.summarise() targets flattened to 720 values
 (2 strains x 120 days x (mean, q10, q90) of log(1 + incidence)).
 """
+import argparse
 import ctypes as C
+import json
 import math
+import os
 
 import numpy as np
 
@@ -226,3 +229,51 @@ def screen(surrogate, X, train_X, thresholds, top=0.01, days=120):
     resim = disagree | outside | top_rows
     return {"mean": mean, "spread": spread, "distance": dist, "outside": outside, "disagree": disagree,
             "top": top_rows, "resimulate": resim, "attack": attack}
+
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate_ensemble.npz")
+SCHEDULE = ((8000, 1e-3), (8000, 3e-4), (8000, 1e-4))
+
+
+def main():
+    """Trains the 5-member ensemble on a training set (the last `heldout`
+    scenarios held out, every other one also strain-swapped) and prints one
+    JSON line: steps, ms per step, TFLOP/s, held-out median errors.
+
+        python -m pandemicthrust_amd.surrogate [--data surrogate_ensemble.npz] [--scale 1.0] [--eager]
+
+    --scale multiplies every phase of the learning-rate schedule's step
+    counts (8000 steps each at 1e-3, 3e-4, 1e-4); --eager enqueues each step
+    kernel by kernel instead of replaying a HIP graph (same results; for
+    profilers that trace launches one by one)."""
+    from .ensemble import load_training_set
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--data", default=DATA)
+    ap.add_argument("--heldout", type=int, default=256)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--eager", action="store_true")
+    a = ap.parse_args()
+    X, Y, meta = load_training_set(a.data)
+    ntr = len(X) - a.heldout
+    sur = Surrogate(members=5, batch=1024, seed=1)
+    sur.use_graphs(not a.eager)
+    Xa, Ya = augment(X[:ntr], Y[:ntr])
+    sur.set_data(normalize(Xa), targets(Ya))
+    steps, ms = 0, 0.0
+    for n, lr in SCHEDULE:
+        n = max(1, int(round(n * a.scale)))
+        sur.train(n, lr=lr, seed=n)
+        steps += n
+        ms += sur.last_train_ms()
+    P = sur.predict(normalize(X[ntr:])).mean(axis=0).reshape(len(X) - ntr, 2, Y.shape[2], 3)
+    sur.close()
+    out = {"steps": steps, "ms_per_step": round(ms / steps, 4),
+           "tflops": round(training_flops(1024, 5) * steps / (ms * 1e-3) / 1e12, 2)}
+    for k, name in enumerate(("pandemic", "seasonal")):
+        e = curve_errors(P[:, k, :, 0], Y[ntr:, k, :, 0])
+        out[name] = {m: round(float(np.median(v)), 4) for m, v in e.items()}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
