@@ -203,9 +203,9 @@ static int probe_values(uint32_t* out, int cap) {
 static void write_schedules(const std::string& dir) {
     struct Cfg { const char* name; float lscale; int core_seed; uint64_t wp_off; uint64_t er_off; };
     const Cfg cfgs[] = {
-        {"c0", 0.0425f, 1, 10625ull, 45630ull},
+        {"small_100k", 0.0425f, 1, 10625ull, 45630ull},
         {"default", 1.0f, 1, 250000ull, 838000ull},
-        {"c1", 21.24f, 1, 5310000ull, 17809740ull},
+        {"full_50M", 21.24f, 1, 5310000ull, 17809740ull},
         {"tiny", 0.001f, 7, 0ull, 1ull},
         {"wrap", 3.0f, 12345, 4294967290ull, 4294967200ull},
     };

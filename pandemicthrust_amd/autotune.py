@@ -91,8 +91,8 @@ def main(argv=None):
     ap.add_argument("--out", default=None, help="write the chosen settings as JSON here")
     a = ap.parse_args(argv)
     from .api import PandemicBatch, Params
-    from .scenarios import load_config2
-    rows = load_config2()[:a.rows]
+    from .scenarios import load_replicate_set
+    rows = load_replicate_set()[:a.rows]
     b = PandemicBatch([Params.from_row(f) for _, f in rows], [s for s, _ in rows])
     b.setup()
     best, log = tune_batch(b)

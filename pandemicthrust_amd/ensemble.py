@@ -17,7 +17,7 @@ R in [1.0, 2.5]).  Scenario vectors are drawn from one std::mt19937_64 stream
     python -m pandemicthrust_amd.ensemble --scenarios 64 --replicates 8 --agents 100000 --days 120 --out ens.npz
 
 The surrogate's training set (4,096 scenarios x 64 replicates at 1,000,000
-agents x 120 days, vectors from seed C3_SEED) is generated in parts and
+agents x 120 days, vectors from seed TRAINING_SET_SEED) is generated in parts and
 merged into pandemicthrust_amd/data/surrogate_ensemble.npz:
 
     python -m pandemicthrust_amd.ensemble --scenarios 4096 --replicates 64 --agents 1000000 --days 120 \
@@ -179,8 +179,8 @@ def run_ensemble(vectors, replicates, n_agents, days, seed0=1, max_agents_per_ba
     return (Y, info, raw) if return_raw else (Y, info)
 
 
-C3_SEED = 20261204          # scenario vectors of the surrogate's training set
-C3_SEED0 = 1                # replicate seeds: C3_SEED0 + scenario * replicates + r
+TRAINING_SET_SEED = 20261204          # scenario vectors of the surrogate's training set
+TRAINING_SET_SEED0 = 1                # replicate seeds: TRAINING_SET_SEED0 + scenario * replicates + r
 
 
 def save_part(path, X, Y, first, info, replicates, agents, days, seed):

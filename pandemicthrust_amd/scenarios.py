@@ -1,6 +1,6 @@
 """Replicate scenario sets for the batch engine.
 
-config2: 256 constants rows for a replicate ensemble on one GPU.  Replicate r
+The replicate set: 256 constants rows for a replicate ensemble on one GPU.  Replicate r
 (r = 0..255) has core seed r + 1, max_days 180 and four draws u1..u4 in that
 order from one std::mt19937_64 stream seeded with 20261202, each draw taken
 as u = (x >> 11) * 2^-53:
@@ -11,7 +11,7 @@ as u = (x >> 11) * 2^-53:
     init_p = 5 + floor(46 u3), init_s = 5 + floor(46 u4)   in 5..50
     percent_symptomatic = 1, asymp_factor = 0       (the reference's constants.csv)
 
-The rows are written to data/config2_scenarios.csv (python -m
+The rows are written to data/replicate_scenarios_256.csv (python -m
 pandemicthrust_amd.scenarios writes it); the realised population of a row is
 what setup draws for it (within about 0.3% of N_r).
 """
@@ -19,8 +19,8 @@ import math
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-CONFIG2_CSV = os.path.join(HERE, "data", "config2_scenarios.csv")
-CONFIG2_SEED = 20261202
+REPLICATE_SET_CSV = os.path.join(HERE, "data", "replicate_scenarios_256.csv")
+REPLICATE_SET_SEED = 20261202
 HEADER = ("replicate,seed,max_days,reproduction_number_pandemic,reproduction_number_seasonal,"
           "initial_infected_pandemic,initial_infected_seasonal,people_scaling_factor,location_scaling_factor,"
           "percent_symptomatic,asymp_factor")
@@ -68,9 +68,9 @@ class MT19937_64:
         return (self() >> 11) * (1.0 / 9007199254740992.0)
 
 
-def config2_rows(n=256):
+def replicate_set_rows(n=256):
     """[(core_seed, [9 constants fields as text]), ...]."""
-    g = MT19937_64(CONFIG2_SEED)
+    g = MT19937_64(REPLICATE_SET_SEED)
     lo, hi = math.log(5e4), math.log(2e6)
     rows = []
     for r in range(n):
@@ -83,14 +83,14 @@ def config2_rows(n=256):
     return rows
 
 
-def config2_text():
+def replicate_set_text():
     lines = [HEADER]
-    for r, (seed, f) in enumerate(config2_rows()):
+    for r, (seed, f) in enumerate(replicate_set_rows()):
         lines.append("%d,%d,%s" % (r, seed, ",".join(f)))
     return "\n".join(lines) + "\n"
 
 
-def load_config2(path=CONFIG2_CSV):
+def load_replicate_set(path=REPLICATE_SET_CSV):
     """Reads the committed rows: [(core_seed, fields), ...]."""
     rows = []
     with open(path) as f:
@@ -103,7 +103,7 @@ def load_config2(path=CONFIG2_CSV):
 
 
 if __name__ == "__main__":
-    os.makedirs(os.path.dirname(CONFIG2_CSV), exist_ok=True)
-    with open(CONFIG2_CSV, "w") as f:
-        f.write(config2_text())
-    print("wrote", CONFIG2_CSV)
+    os.makedirs(os.path.dirname(REPLICATE_SET_CSV), exist_ok=True)
+    with open(REPLICATE_SET_CSV, "w") as f:
+        f.write(replicate_set_text())
+    print("wrote", REPLICATE_SET_CSV)

@@ -1,7 +1,7 @@
 """Writes whole-run fixtures of the CPU oracle for workloads too large to run
 the oracle inside a test: the 50M-agent x 100-day workload
 (constants,100,1.8,1.3,10,10,21.24,21.24,1,0, seed 1) and the largest rows of
-the 256-scenario replicate set (pandemicthrust_amd/data/config2_scenarios.csv,
+the 256-scenario replicate set (pandemicthrust_amd/data/replicate_scenarios_256.csv,
 about 2M agents x 180 days).
 
 Each fixture directory holds what the reference's executable writes
@@ -12,9 +12,9 @@ each of the six final per-agent arrays (status, day, generation per strain,
 uint8).  The GPU tests compare a run of the HIP engine with them byte for
 byte.
 
-  python tests/golden/make_fullscale_fixture.py c1
-  python tests/golden/make_fullscale_fixture.py c2 ROW [ROW ...]
-  python tests/golden/make_fullscale_fixture.py c2-largest K
+  python tests/golden/make_fullscale_fixture.py loop
+  python tests/golden/make_fullscale_fixture.py replicate ROW [ROW ...]
+  python tests/golden/make_fullscale_fixture.py replicate-largest K
 
 Runs single-threaded; the 50M x 100-day run takes several minutes and about
 6 GB of memory.
@@ -32,8 +32,8 @@ sys.path.insert(0, os.path.dirname(HERE))
 
 import oracle_ctypes as O  # noqa: E402
 
-C1_ROW = ["100", "1.8", "1.3", "10", "10", "21.24", "21.24", "1", "0"]
-SCENARIOS = os.path.join(ROOT, "pandemicthrust_amd", "data", "config2_scenarios.csv")
+LOOP_ROW = ["100", "1.8", "1.3", "10", "10", "21.24", "21.24", "1", "0"]
+SCENARIOS = os.path.join(ROOT, "pandemicthrust_amd", "data", "replicate_scenarios_256.csv")
 ARRAY_NAMES = ["status_p", "status_s", "day_p", "day_s", "gen_p", "gen_s"]
 
 
@@ -87,20 +87,20 @@ def main(argv):
     if not argv:
         print(__doc__)
         return 2
-    if argv[0] == "c1":
-        write_fixture("c1_50M_100d", C1_ROW, 1)
+    if argv[0] == "loop":
+        write_fixture("loop_50M_100d", LOOP_ROW, 1)
         return 0
     rows = scenario_rows()
-    if argv[0] == "c2-largest":
+    if argv[0] == "replicate-largest":
         k = int(argv[1])
         pick = sorted(rows, key=lambda t: -float(t[1][5]))[:k]
-    elif argv[0] == "c2":
+    elif argv[0] == "replicate":
         pick = [rows[int(x)] for x in argv[1:]]
     else:
         print(__doc__)
         return 2
     for i, fields, seed in pick:
-        write_fixture("c2_row%03d" % i, fields, seed)
+        write_fixture("replicate_row%03d" % i, fields, seed)
     return 0
 
 

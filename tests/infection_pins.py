@@ -222,7 +222,7 @@ class InfectionPins:
                 self._fail("day %d strain %d: %d slots report an infection they did not win"
                            % (d, s, int((reported != want).sum())))
 
-    # ---- verdicts ----
+    # ---- results ----
     def cell_z(self, min_expected=10.0):
         out = []
         for key, (n, o, e, v) in sorted(self.cells.items()):

@@ -23,7 +23,7 @@ import pytest
 
 from pandemicthrust_amd import PandemicBatch, PandemicSim, Params
 from pandemicthrust_amd import outputs
-from pandemicthrust_amd.scenarios import load_config2
+from pandemicthrust_amd.scenarios import load_replicate_set
 
 pytestmark = pytest.mark.gpu
 
@@ -76,11 +76,11 @@ def _solo_check(name):
 
 
 def test_50M_agents_100_days_equal_the_oracle():
-    _solo_check("c1_50M_100d")
+    _solo_check("loop_50M_100d")
 
 
 def test_50M_agents_host_program_files_equal_the_oracle(tmp_path):
-    st, files = fixture("c1_50M_100d")
+    st, files = fixture("loop_50M_100d")
     d = str(tmp_path)
     with open(os.path.join(d, "constants.csv"), "w") as f:
         f.write(HEADER + "\n" + st["constants"] + "\n")
@@ -93,23 +93,23 @@ def test_50M_agents_host_program_files_equal_the_oracle(tmp_path):
             assert f.read() == files[n], n
 
 
-def _c2_fixtures():
-    return sorted(os.path.basename(p) for p in glob.glob(os.path.join(GOLDEN, "c2_row*")))
+def _replicate_fixtures():
+    return sorted(os.path.basename(p) for p in glob.glob(os.path.join(GOLDEN, "replicate_row*")))
 
 
-@pytest.mark.parametrize("name", _c2_fixtures())
+@pytest.mark.parametrize("name", _replicate_fixtures())
 def test_largest_scenarios_equal_the_oracle(name):
     _solo_check(name)
 
 
 def test_256_scenario_batch_equals_solo_runs_and_the_oracle():
-    rows = load_config2()
+    rows = load_replicate_set()
     assert len(rows) == 256
     print("case: 256-row replicate batch", flush=True)
     b = PandemicBatch([Params.from_row(f) for _, f in rows], [s for s, _ in rows])
     n = b.setup()
     res = b.run()
-    fixtures = {int(name[len("c2_row"):]): fixture(name) for name in _c2_fixtures()}
+    fixtures = {int(name[len("replicate_row"):]): fixture(name) for name in _replicate_fixtures()}
     assert len(fixtures) >= 2
     for r, (seed, f) in enumerate(rows):
         c, inf = res[r]

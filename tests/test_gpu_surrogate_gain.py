@@ -63,7 +63,7 @@ def test_trained_surrogate_gain_over_simulation():
     # The method without the surrogate: simulate the scenarios' 64 replicates.
     pick = [ntr, ntr + 1]
     t0 = time.perf_counter()
-    Ysim, info = E.run_ensemble(X[pick], 64, 1_000_000, 120, seed0=E.C3_SEED0 + ntr * 64)
+    Ysim, info = E.run_ensemble(X[pick], 64, 1_000_000, 120, seed0=E.TRAINING_SET_SEED0 + ntr * 64)
     sim_s = (time.perf_counter() - t0) / len(pick)
     assert np.array_equal(Ysim.astype(np.float16), Y[pick].astype(np.float16))
     gain = sim_s / (pred_s / HELDOUT)

@@ -105,8 +105,8 @@ def test_scaling_matches_reference(cfg):
 def test_scaling_closed_form_sizes():
     # W at the two benchmark configs (every type count rounded half up,
     # min 1 where the base count is > 0): 545 and 271,872.
-    assert CFG["c0"][1] == 545
-    assert CFG["c1"][1] == 271872
+    assert CFG["small_100k"][1] == 545
+    assert CFG["full_50M"][1] == 271872
 
 
 @pytest.mark.parametrize("cfg", sorted(CFG))

@@ -24,11 +24,11 @@ def test_mt19937_64_seeded_stream():
 
 
 def test_committed_csv_is_the_generator_output():
-    assert open(S.CONFIG2_CSV).read() == S.config2_text()
+    assert open(S.REPLICATE_SET_CSV).read() == S.replicate_set_text()
 
 
-def test_config2_ranges():
-    rows = S.load_config2()
+def test_replicate_set_ranges():
+    rows = S.load_replicate_set()
     assert len(rows) == 256
     assert [s for s, _ in rows] == list(range(1, 257))
     for seed, f in rows:

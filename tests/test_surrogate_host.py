@@ -127,5 +127,5 @@ def test_committed_training_set_is_the_stated_one():
     X, Y, meta = E.load_training_set(S.DATA)
     assert X.shape == (8192, 16) and Y.shape == (8192, 2, 120, 3)
     assert meta["replicates"] == 64 and meta["agents"] == 1_000_000 and meta["days"] == 120
-    assert np.array_equal(X, E.sample_scenarios(8192, E.C3_SEED))
+    assert np.array_equal(X, E.sample_scenarios(8192, E.TRAINING_SET_SEED))
     assert np.isfinite(Y).all() and Y.min() >= 0.0
