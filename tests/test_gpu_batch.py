@@ -15,8 +15,8 @@ pytestmark = pytest.mark.gpu
 
 # Diverse replicates: population scale, location scale, R, seeds, initial
 # infected, symptomatic share, asymptomatic factor and max_days all vary;
-# one row has R = 0 (extinction on day 10) and one stops at max_days while
-# still infected.
+# one row has R = 0 (extinction on day 10), one stops at max_days while
+# still infected, and one has no initial infected (it ends on day 0).
 ROWS = [
     (["40", "1.8", "1.3", "10", "10", "0.0425", "0.0425", "1", "0"], 1),
     (["35", "2.4", "1.1", "5", "30", "0.021", "0.021", "1", "0"], 2),
@@ -26,6 +26,7 @@ ROWS = [
     (["45", "1.9", "1.6", "7", "9", "0.0213", "0.0213", "0.9", "0.3"], 6),
     (["30", "1.8", "1.3", "10", "10", "0.08", "0.08", "1", "0"], 7),
     (["50", "3.0", "2.0", "1", "1", "0.025", "0.04", "1", "0"], 8),
+    (["20", "1.8", "1.3", "0", "0", "0.03", "0.03", "1", "0"], 9),    # nobody infected: ends on day 0
 ]
 
 STATE = ["status_p", "status_s", "day_p", "day_s", "gen_p", "gen_s"]
@@ -86,6 +87,9 @@ def test_batch_run_matches_single_simulations(mode):
     # R = 0: only the initial infected, recovered on day 10.
     c, inf = res[3]
     assert len(inf) == 11 and inf[-1] == 0 and inf[0] == 20
+    # no initial infected: one day, nobody infected, everybody susceptible
+    c, inf = res[8]
+    assert len(inf) == 1 and inf[0] == 0 and c[0][1] == n[8] and c[0][9] == n[8]
     b.close()
 
 

@@ -72,6 +72,9 @@ def test_day_by_day_bit_exact():
     (["40", "1.8", "1.3", "10", "10", "0.3", "0.3", "1", "0"], 1),            # ~700k agents
     (["20", "3.0", "2.0", "3", "3", "0.0005", "0.0005", "1", "0"], 11),       # 500 households, min-1 locations
     (["45", "1.8", "1.3", "10", "10", "0.0425", "0.5", "1", "0"], 2),          # many small locations
+    (["20", "1.8", "1.3", "0", "0", "0.0425", "0.0425", "1", "0"], 4),          # nobody infected: ends on day 0
+    (["30", "1.8", "1.3", "0", "12", "0.0425", "0.0425", "1", "0"], 5),         # seasonal strain only
+    (["30", "2.0", "1.3", "15", "0", "0.0425", "0.0425", "1", "0"], 6),         # pandemic strain only
 ])
 def test_full_run_bit_exact(row, seed):
     gpu, ref = _sims(row, seed)
