@@ -44,7 +44,21 @@ __device__ __forceinline__ int32_t implicit_id(int64_t j, int mode, int32_t N) {
     return (int32_t)j;
 }
 
+// The same without branches (selects only), for the downsweep's unrolled
+// loads: a branch there makes the compiler wait for the loads in flight.
+// j / 3 as (j * 0xAAAAAAAB) >> 33, exact for every j < 2^32 (pair
+// positions are below 2^32).
+__device__ __forceinline__ int32_t implicit_id_select(int64_t j, int mode, int32_t N) {
+    const int32_t hm = (int32_t)(j < N ? j : j - N);
+    const int32_t d3 = (int32_t)(((uint64_t)(uint32_t)j * 0xAAAAAAABull) >> 33);
+    return mode == ID_DIV3 ? d3 : (mode == ID_HOUR_MAJOR ? hm : (int32_t)j);
+}
+
 __device__ __forceinline__ bool is_done(const int32_t* done) { return done && *done; }
+
+// Downsweep input: interleaved pairs, keys with ids implied by the position,
+// or keys with explicit ids.
+enum DownIn : int { IN_PAIRS = 0, IN_KEYS = 1, IN_KEYS_VALS = 2 };
 
 // The block's slice of the n pairs (n from the device when n_dev is set),
 // whole tiles of T * kItems pairs.
@@ -206,10 +220,12 @@ __global__ __launch_bounds__(T) void k_upsweep(const uint32_t* __restrict__ keys
 // counter with a returning atomic, so the 8 rounds of a wave issue back to
 // back and LDS order gives each round the count of the rounds before it.
 // The tile is then staged in LDS in digit order and written out as runs per
-// digit, while the next tile's pairs are already in flight.  Input: keys
-// plus explicit or implied ids (PIN false) or pairs (PIN true); output:
+// digit, while the next tile's pairs are already in flight.  Input (IN):
+// keys with ids implied by the position, keys with explicit ids, or pairs;
+// each is its own instantiation, so the unrolled loads are branch-free and
+// stay in flight; output:
 // pairs (POUT true) or separate keys and ids.
-template <int BITS, bool PIN, bool POUT, int T>
+template <int BITS, int IN, bool POUT, int T>
 __global__ __launch_bounds__(T) void k_downsweep(const uint32_t* __restrict__ keys_in,
                                                  const int32_t* __restrict__ vals_in, const uint2* __restrict__ pairs_in,
                                                  int id_mode, int32_t N, uint32_t* __restrict__ keys_out,
@@ -219,6 +235,7 @@ __global__ __launch_bounds__(T) void k_downsweep(const uint32_t* __restrict__ ke
                                                  const int32_t* done, uint32_t drop, uint32_t* __restrict__ n_out,
                                                  SegLayout seg) {
     if (is_done(done)) return;
+    constexpr bool PIN = IN == IN_PAIRS;
     constexpr int RADIX = 1 << BITS;
     constexpr int W = T / 64;
     constexpr int TILE = T * kItems;
@@ -255,7 +272,7 @@ __global__ __launch_bounds__(T) void k_downsweep(const uint32_t* __restrict__ ke
             v = (int32_t)p.y;
         } else {
             k = valid ? keys_in[j] : 0u;
-            v = vals_in ? (valid ? vals_in[j] : 0) : implicit_id(j, id_mode, N);
+            v = IN == IN_KEYS_VALS ? (valid ? vals_in[j] : 0) : implicit_id_select(j, id_mode, N);
         }
     };
 
@@ -558,13 +575,17 @@ hipError_t radix_pass(TableScratch& s, const PassIO& io, int id_mode, int32_t N,
     launch(tm, KC_SORT_DOWNSWEEP, k_downsweep<BITS, PI, PO, T>, dim3(nblk), dim3(T), st, io.kin, io.vin, io.pin,      \
            id_mode, N, io.kout, io.vout, io.pout, h, n, n_dev, shift, nblk, done, drop, n_out, seg)
     if (pin && io.pout)
-        PT_DOWN(true, true);
+        PT_DOWN(IN_PAIRS, true);
     else if (pin)
-        PT_DOWN(true, false);
+        PT_DOWN(IN_PAIRS, false);
+    else if (io.vin && io.pout)
+        PT_DOWN(IN_KEYS_VALS, true);
+    else if (io.vin)
+        PT_DOWN(IN_KEYS_VALS, false);
     else if (io.pout)
-        PT_DOWN(false, true);
+        PT_DOWN(IN_KEYS, true);
     else
-        PT_DOWN(false, false);
+        PT_DOWN(IN_KEYS, false);
 #undef PT_DOWN
     return hipGetLastError();
 }
