@@ -581,24 +581,33 @@ int pt_mlp_train(pt_mlp* m, const int32_t* order, int32_t steps, float lr, float
                                                                m->d_step);
         k_step_done<<<1, 1, 0, m->stream>>>(m->d_step);
     };
-    hipGraphExec_t exec = nullptr;
+    // The instantiated step graph, released on every return.
+    struct ExecGuard {
+        hipGraphExec_t e = nullptr;
+        ~ExecGuard() {
+            if (e) (void)hipGraphExecDestroy(e);
+        }
+    } exec;
     if (m->use_graphs) {
         hipGraph_t graph = nullptr;
         HIP_TRY(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
         one_step();
-        HIP_TRY(hipStreamEndCapture(m->stream, &graph));
-        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        // Ends the capture whatever happened inside it; a failed launch
+        // there is reported here.
+        const hipError_t ce = hipStreamEndCapture(m->stream, &graph);
+        if (ce != hipSuccess || !graph) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(PT_EDEVICE, "training graph capture: %s", hipGetErrorString(ce));
+        }
+        const hipError_t e = hipGraphInstantiate(&exec.e, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) return fail(PT_EDEVICE, "training graph: %s", hipGetErrorString(e));
     }
     HIP_TRY(hipEventRecord(m->ev0, m->stream));
     for (int s = 0; s < steps; ++s) {
-        if (exec) {
-            const hipError_t e = hipGraphLaunch(exec, m->stream);
-            if (e != hipSuccess) {
-                (void)hipGraphExecDestroy(exec);
-                return fail(PT_EDEVICE, "training graph launch: %s", hipGetErrorString(e));
-            }
+        if (exec.e) {
+            const hipError_t e = hipGraphLaunch(exec.e, m->stream);
+            if (e != hipSuccess) return fail(PT_EDEVICE, "training graph launch: %s", hipGetErrorString(e));
         } else {
             one_step();
         }
@@ -607,7 +616,6 @@ int pt_mlp_train(pt_mlp* m, const int32_t* order, int32_t steps, float lr, float
     HIP_TRY(hipEventRecord(m->ev1, m->stream));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventSynchronize(m->ev1));
-    if (exec) (void)hipGraphExecDestroy(exec);
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
     m->last_ms = ms;
