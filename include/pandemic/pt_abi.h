@@ -281,7 +281,10 @@ int pt_sim_get_launch(pt_sim* sim, int32_t knob);
  * offsets[n_buckets] = n).  All three are device pointers on `device`;
  * stream is a hipStream_t (NULL: the default stream).  The radix scratch
  * (about 20 bytes per pair) is allocated for the call and freed before it
- * returns, so the call synchronises the stream.  n < 2^31, n_buckets >= 1. */
+ * returns, so the call synchronises the stream.  n < 2^31, n_buckets >= 1.
+ * Keys are not checked on the device: a key >= n_buckets leaves the table
+ * unspecified, but every write stays inside people[n] and
+ * offsets[n_buckets + 1]. */
 int pt_location_table(const uint32_t* keys, int64_t n, int32_t n_buckets, int32_t* people, int32_t* offsets,
                       int32_t device, void* stream);
 
