@@ -160,7 +160,8 @@ int or_get_trace_draws(or_sim* s, uint8_t* draw, uint8_t* profile);
  * calibrated tables (valid after or_create). */
 uint8_t or_infection_profile(or_sim* s, uint32_t r);
 float or_infection_probability(or_sim* s, int profile, int dd, int strain, int kval_sum);
-/* Which deliberate mistake this build carries (0: none; see pt_oracle.cpp). */
+/* Which deliberate mistake this compiled oracle carries (0: none; see
+ * pt_oracle.cpp). */
 int or_plant(void);
 
 /* Writes the reference's output files into dir. */
