@@ -4,7 +4,9 @@ at its full size.
 - Setup: every population table equals the oracle's, bit-exact (the oracle's
   setup takes a few seconds at this size).
 - The first four days equal the oracle's day by day (counts, infected list,
-  all agent state).
+  all agent state), and so does the first week under every intervention at
+  once (latent and infectious periods, a school closure over days 2-5,
+  workplace and errand reductions, vaccination; 3,000 seeds).
 - The day's errand tables, built by the radix sort over 100M (weekday) and
   150M (weekend) pairs, checked through size-independent properties against
   the errand hook (people_errands): every entry sits in the bucket of its
@@ -52,6 +54,30 @@ def test_setup_and_first_days_equal_the_oracle():
         assert np.array_equal(gpu.array(PT_INFECTED), ref.array(O.OR_INFECTED)), day
         for k, (a, b) in enumerate(zip(_state(gpu), [ref.array(w) for w in range(6)])):
             assert np.array_equal(a, b), (day, k)
+    gpu.close()
+
+
+def test_first_week_with_interventions_equals_the_oracle():
+    from pandemicthrust_amd import Interventions
+    row = ["100", "1.8", "1.3", "2000", "1000", "21.24", "21.24", "1", "0"]
+    iv = Interventions(latent=(1, 0), infectious=(8, 6), school_close_start=2, school_close_days=4,
+                       workplace_reduction=0.3, errand_reduction=0.2, vaccination=(0.15, 0.05))
+    gpu = PandemicSim(Params.from_row(row), SEED)
+    gpu.set_interventions(iv)
+    ref = O.OracleSim(O.params_from_row(row), SEED)
+    ref.set_interventions(iv)
+    n = gpu.setup_sim()
+    ref.setup()
+    assert n == ref.n_people > 49_000_000
+    for day in range(7):   # five weekdays (closure on days 2-5) and a weekend
+        print("interventions at 50M: day %d" % day, flush=True)
+        rg, cg, ig = gpu.step()
+        rr, cr, ir = ref.step()
+        assert (rg, ig) == (rr, ir) and np.array_equal(cg, cr), day
+        assert np.array_equal(gpu.array(PT_INFECTED), ref.array(O.OR_INFECTED)), day
+        for k, (a, b) in enumerate(zip(_state(gpu), [ref.array(w) for w in range(6)])):
+            assert np.array_equal(a, b), (day, k)
+    assert ig > 3000      # the epidemic grew under the interventions
     gpu.close()
 
 
