@@ -1,8 +1,13 @@
-"""The replicate batch at its largest supported size: 14 runs of the
-50M-agent workload take 700,055,552 of the 715,827,882 agent slots a batch
-holds (each run rounded up to 8192-slot tiles; include/pandemic/pt_batch.h).
-Over 3 weekdays the first and last replicates equal their single runs day
-by day, and a 15th replicate is refused with an error rather than run."""
+"""The replicate batch at large sizes.
+
+- At its largest supported size: 14 runs of the 50M-agent workload take
+  700,055,552 of the 715,827,882 agent slots a batch holds (each run rounded
+  up to 8192-slot tiles; include/pandemic/pt_batch.h).  Over 3 weekdays the
+  first and last replicates equal their single runs day by day, and a 15th
+  replicate is refused with an error rather than run.
+- As the surrogate sweep re-simulates: two intervention scenarios at 50M
+  agents (initial infected per million), two replicates each, in one batch,
+  equal their single runs over a week, weekend included."""
 import numpy as np
 import pytest
 
@@ -45,3 +50,29 @@ def test_batch_beyond_the_slot_limit_is_refused():
     with pytest.raises(Exception, match="exceeds the supported total"):
         b.setup()
     b.close()
+
+
+def test_sweep_resimulation_batch_matches_single_runs():
+    from pandemicthrust_amd import ensemble as E
+    vecs = E.sample_scenarios(2, 303)
+    params, ivs, seeds = [], [], []
+    for i, v in enumerate(vecs):
+        p, iv = E.scenario_inputs(v, 50_000_000, 7, seeds_per_million=True)
+        for r in range(2):
+            params.append(p)
+            ivs.append(iv)
+            seeds.append(5_000_000 + 2 * i + r)
+    print("batch: 2 scenarios x 2 replicates x 50M agents, 7 days", flush=True)
+    b = PandemicBatch(params, seeds)
+    b.set_interventions(ivs)
+    b.setup()
+    res = b.run()
+    b.close()
+    for j in range(len(params)):
+        print("single run %d" % j, flush=True)
+        s = PandemicSim(params[j], seeds[j])
+        s.set_interventions(ivs[j])
+        s.setup_sim()
+        counts, inf = s.run()
+        s.close()
+        assert np.array_equal(res[j][0], counts) and np.array_equal(res[j][1], inf), j
