@@ -8,6 +8,8 @@
   generation offsets and the SHA-256 of all six final per-agent arrays equal
   the oracle's (runToCompletion, PandemicSim.cu:877-967; the files of
   :2771-2818, :2282-2295, :863).
+- The same run saved at day 60 (pt_sim_checkpoint_save) and resumed in a
+  fresh handle: identical to the fixture.
 - The largest rows of the replicate set (about 2M agents x 180 days): the
   same, solo and as replicates of the full 256-row batch.
 - All 256 rows batched: every replicate equal to its own solo run.
@@ -77,6 +79,29 @@ def _solo_check(name):
 
 def test_50M_agents_100_days_equal_the_oracle():
     _solo_check("loop_50M_100d")
+
+
+def test_50M_agents_resumed_mid_run_equal_the_oracle():
+    """Saved at the day-60 boundary (rising towards the peak), loaded into a
+    fresh handle and run to the end: the same files, tallies and final
+    state as the oracle's uninterrupted run."""
+    st, files = fixture("loop_50M_100d")
+    print("case: loop_50M_100d resumed from day 60", flush=True)
+    a = PandemicSim(Params.from_row(_row(st)), st["seed"])
+    n = a.setup_sim()
+    for _ in range(60):
+        a.step()
+    blob = a.save_checkpoint()
+    a.close()
+    b = PandemicSim(Params.from_row(_row(st)), st["seed"])
+    b.setup_only()
+    b.load_checkpoint(blob)
+    del blob
+    b.run()
+    counts, inf = b.daily_rows()
+    cp, cs = b.generation_offsets()
+    _check_run("loop_50M_100d resumed", st, files, counts, inf, n, cp, cs, [b.array(k) for k in range(6)])
+    b.close()
 
 
 def test_50M_agents_host_program_files_equal_the_oracle(tmp_path):
