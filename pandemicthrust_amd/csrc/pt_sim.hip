@@ -985,11 +985,6 @@ int enqueue_day(pt_sim* s, int d) {
     else
         launch(tm, KC_GEN_WEEKDAY, k_gen_weekday, dim3(s->grid_gen), dim3(256), stc, M, st, (const uint8_t*)s->age,
                s->keys);
-    // Weekday: the last bucket collects the hour-1 keys nobody queries
-    // (children stay at their after-school site), left out of the table.
-    hipError_t e = build_table(s->tables, s->keys, nullptr, n, nullptr, K, mode, N, s->er_people, s->er_off,
-                               &st->skip_full, stc, tm, weekend ? kNoDrop : (uint32_t)(K - 1));
-    if (e != hipSuccess) return fail(PT_EDEVICE, "table build: %s", hipGetErrorString(e));
     // Subset path: only pairs of queried buckets, explicit ids.
     const int nb = (int)((s->N + (weekend ? kSelAgentsWe : kSelAgentsWd) - 1) / (weekend ? kSelAgentsWe : kSelAgentsWd));
     if (weekend)
@@ -998,17 +993,27 @@ int enqueue_day(pt_sim* s, int d) {
     else
         launch(tm, KC_SELECT, k_select_weekday, dim3(nb), dim3(kSelThreads), stc, M, st, (const uint8_t*)s->age,
                (const uint32_t*)s->mark, s->stage_keys, s->stage_ids, s->sblk_cnt);
-    e = exclusive_scan_u32(s->sblk_cnt, s->sblk_off, nb, s->tables.scan_part, &st->skip_subset, stc, tm, KC_SELECT);
+    hipError_t e =
+        exclusive_scan_u32(s->sblk_cnt, s->sblk_off, nb, s->tables.scan_part, &st->skip_subset, stc, tm, KC_SELECT);
     if (e != hipSuccess) return fail(PT_EDEVICE, "selection scan: %s", hipGetErrorString(e));
     launch(tm, KC_SELECT, k_set_nsel, dim3(1), dim3(1), stc, st, (const uint32_t*)s->sblk_cnt,
            (const uint32_t*)s->sblk_off, nb);
     launch(tm, KC_SELECT, k_compact_runs, dim3(nb), dim3(256), stc, (const int32_t*)&st->skip_subset,
            (const uint32_t*)s->stage_keys, (const int32_t*)s->stage_ids, (const uint32_t*)s->sblk_cnt,
            (const uint32_t*)s->sblk_off, s->sel_keys, s->sel_ids, weekend ? kSelStageWe : kSelStageWd);
-    ClassTimer subset_tm(tm, KC_SUBSET_SORT);
-    e = build_table(s->tables, s->sel_keys, s->sel_ids, n, &st->n_sel, K, mode, N, s->er_people, s->er_off,
-                    &st->skip_subset, stc, &subset_tm);
-    if (e != hipSuccess) return fail(PT_EDEVICE, "subset table build: %s", hipGetErrorString(e));
+    // One table build serves both paths: its first pass reads the selected
+    // pairs on subset days (skip_full set) and every agent's keys otherwise.
+    // Weekday: the last bucket collects the hour-1 keys nobody queries
+    // (children stay at their after-school site), left out of the full table
+    // (the selection never holds it).
+    AltInput sel;
+    sel.flag = &st->skip_full;
+    sel.keys = s->sel_keys;
+    sel.vals = s->sel_ids;
+    sel.n_dev = &st->n_sel;
+    e = build_table(s->tables, s->keys, nullptr, n, nullptr, K, mode, N, s->er_people, s->er_off, &st->done, stc, tm,
+                    weekend ? kNoDrop : (uint32_t)(K - 1), &sel);
+    if (e != hipSuccess) return fail(PT_EDEVICE, "table build: %s", hipGetErrorString(e));
     if (weekend)
         launch(tm, KC_CONTACTS_WEEKEND, k_contacts<true>, dim3(s->grid_contacts), dim3(kContactThreads), stc, M, st,
                infected, A, s->list_p, s->list_s, s->workplace_off);

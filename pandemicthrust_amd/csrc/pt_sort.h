@@ -31,7 +31,7 @@ enum KernelClass : int {
     KC_END_DAY,
     KC_MARK,
     KC_SELECT,
-    KC_SUBSET_SORT,
+    KC_SUBSET_SORT,  // idle: subset and full errand tables share one build (the sort classes); index kept
     KC_SETUP,
     KC_COUNT
 };
@@ -44,15 +44,6 @@ struct LaunchTimer {
     // A fresh (start, stop) event pair accounted to class cls.
     virtual void events(int cls, hipEvent_t* start, hipEvent_t* stop) = 0;
     virtual ~LaunchTimer() {}
-};
-
-// Accounts every launch to one class of another timer.
-struct ClassTimer : LaunchTimer {
-    LaunchTimer* inner;
-    int cls;
-    ClassTimer(LaunchTimer* t, int c) : inner(t), cls(c) {}
-    bool want(int) const override { return inner && inner->want(cls); }
-    void events(int, hipEvent_t* start, hipEvent_t* stop) override { inner->events(cls, start, stop); }
 };
 
 template <typename F, typename... Args>
@@ -98,6 +89,19 @@ void table_scratch_free(TableScratch& s);
 
 constexpr uint32_t kNoDrop = 0xFFFFFFFFu;
 
+// A second input for a table whose input is chosen on the device: while
+// *flag is nonzero the first radix pass reads keys with the explicit ids vals
+// (*n_dev pairs, none dropped) instead of the build's own keys; the later
+// passes and the offsets are the same launches either way, so a caller that
+// decides between two tables on the device enqueues one build instead of
+// two, each idle on the days the other runs.
+struct AltInput {
+    const int32_t* flag = nullptr;
+    const uint32_t* keys = nullptr;
+    const int32_t* vals = nullptr;
+    const uint32_t* n_dev = nullptr;
+};
+
 // Builds the table.  keys is only read.  vals (may be null) holds explicit
 // ids in input order; when null the ids follow from mode.  When n_dev is
 // set it holds the pair count on the device and n is only the capacity.
@@ -105,9 +109,13 @@ constexpr uint32_t kNoDrop = 0xFFFFFFFFu;
 // drop_key = K - 1 leaves the pairs of the last bucket out of the table (a
 // bucket nobody queries): that bucket comes out empty, offsets[K - 1] =
 // offsets[K] = pairs kept, and ts.n_kept holds that count on the device.
+// alt (may be null; requires vals and n_dev null) is the input the device
+// may pick instead (AltInput); ts.n_kept then holds the pair count of
+// whichever input was read, and n must also bound *alt->n_dev.
 hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int64_t n, const uint32_t* n_dev,
                        int32_t K, IdMode mode, int32_t N, int32_t* people_out, int32_t* offsets_out,
-                       const int32_t* done, hipStream_t stream, LaunchTimer* tm, uint32_t drop_key = kNoDrop);
+                       const int32_t* done, hipStream_t stream, LaunchTimer* tm, uint32_t drop_key = kNoDrop,
+                       const AltInput* alt = nullptr);
 
 // Segmented tables: the tables of R independent simulations built by one
 // sort whose passes cover only the bits of the largest local key.  Segment r

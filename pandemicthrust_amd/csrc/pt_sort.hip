@@ -57,8 +57,13 @@ __device__ __forceinline__ int32_t implicit_id_select(int64_t j, int mode, int32
 __device__ __forceinline__ bool is_done(const int32_t* done) { return done && *done; }
 
 // Downsweep input: interleaved pairs, keys with ids implied by the position,
-// or keys with explicit ids.
-enum DownIn : int { IN_PAIRS = 0, IN_KEYS = 1, IN_KEYS_VALS = 2 };
+// keys with explicit ids, or (IN_DUAL, a first pass with an AltInput) keys
+// with implied ids unless the device flag picks the other keys with
+// explicit ids.
+enum DownIn : int { IN_PAIRS = 0, IN_KEYS = 1, IN_KEYS_VALS = 2, IN_DUAL = 3 };
+
+// The first pass of a table with an AltInput reads the other input today.
+__device__ __forceinline__ bool alt_today(const AltInput& a) { return a.flag && *a.flag; }
 
 // The block's slice of the n pairs (n from the device when n_dev is set),
 // whole tiles of T * kItems pairs.
@@ -135,8 +140,13 @@ template <int BITS, bool MATCH, bool PIN, int T>
 __global__ __launch_bounds__(T) void k_upsweep(const uint32_t* __restrict__ keys, const uint2* __restrict__ pairs,
                                                int64_t n_host, const uint32_t* __restrict__ n_dev, int shift,
                                                uint32_t* __restrict__ hist, int nblk, const int32_t* done,
-                                               uint32_t drop, SegLayout seg) {
+                                               uint32_t drop, SegLayout seg, AltInput alt) {
     if (is_done(done)) return;
+    if (!PIN && alt_today(alt)) {
+        keys = alt.keys;
+        n_dev = alt.n_dev;
+        drop = kNoDrop;
+    }
     constexpr int RADIX = 1 << BITS;
     constexpr int W = T / 64;
     constexpr int G = PIN ? 2 : 4;  // keys per 16-byte load
@@ -225,16 +235,27 @@ __global__ __launch_bounds__(T) void k_upsweep(const uint32_t* __restrict__ keys
 // each is its own instantiation, so the unrolled loads are branch-free and
 // stay in flight; output:
 // pairs (POUT true) or separate keys and ids.
+template <int BITS, int T>
+struct DownSmem {
+    uint2 pairs[T * kItems];
+    uint32_t wcnt[T / 64][1 << BITS];
+    uint32_t start[1 << BITS];
+    uint32_t tot[1 << BITS];
+    uint32_t base[1 << BITS];
+    uint32_t delta[1 << BITS];
+    uint32_t w[T / 64];
+    uint32_t kept;
+};
+
 template <int BITS, int IN, bool POUT, int T>
-__global__ __launch_bounds__(T) void k_downsweep(const uint32_t* __restrict__ keys_in,
-                                                 const int32_t* __restrict__ vals_in, const uint2* __restrict__ pairs_in,
-                                                 int id_mode, int32_t N, uint32_t* __restrict__ keys_out,
-                                                 int32_t* __restrict__ vals_out, uint2* __restrict__ pairs_out,
-                                                 const uint32_t* __restrict__ hist, int64_t n_host,
-                                                 const uint32_t* __restrict__ n_dev, int shift, int nblk,
-                                                 const int32_t* done, uint32_t drop, uint32_t* __restrict__ n_out,
-                                                 SegLayout seg) {
-    if (is_done(done)) return;
+__device__ __forceinline__ void downsweep_body(DownSmem<BITS, T>& sm, const uint32_t* __restrict__ keys_in,
+                                               const int32_t* __restrict__ vals_in,
+                                               const uint2* __restrict__ pairs_in, int id_mode, int32_t N,
+                                               uint32_t* __restrict__ keys_out, int32_t* __restrict__ vals_out,
+                                               uint2* __restrict__ pairs_out, const uint32_t* __restrict__ hist,
+                                               int64_t n_host, const uint32_t* __restrict__ n_dev, int shift,
+                                               int nblk, uint32_t drop, uint32_t* __restrict__ n_out,
+                                               const SegLayout& seg) {
     constexpr bool PIN = IN == IN_PAIRS;
     constexpr int RADIX = 1 << BITS;
     constexpr int W = T / 64;
@@ -243,14 +264,14 @@ __global__ __launch_bounds__(T) void k_downsweep(const uint32_t* __restrict__ ke
     if (n_out && blockIdx.x == 0 && threadIdx.x == 0) *n_out = hist[(int64_t)nblk << BITS];
     constexpr int DPT = RADIX > T ? RADIX / T : 1;  // digits per thread in the digit scan
     constexpr uint32_t MASK = RADIX - 1;
-    __shared__ uint2 s_pairs[TILE];
-    __shared__ uint32_t s_wcnt[W][RADIX];
-    __shared__ uint32_t s_start[RADIX];
-    __shared__ uint32_t s_tot[RADIX];
-    __shared__ uint32_t s_base[RADIX];
-    __shared__ uint32_t s_delta[RADIX];
-    __shared__ uint32_t s_w[W];
-    __shared__ uint32_t s_kept;
+    uint2* s_pairs = sm.pairs;
+    auto& s_wcnt = sm.wcnt;
+    uint32_t* s_start = sm.start;
+    uint32_t* s_tot = sm.tot;
+    uint32_t* s_base = sm.base;
+    uint32_t* s_delta = sm.delta;
+    uint32_t* s_w = sm.w;
+    uint32_t& s_kept = sm.kept;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int64_t begin, end;
@@ -405,6 +426,34 @@ __global__ __launch_bounds__(T) void k_downsweep(const uint32_t* __restrict__ ke
     }
 }
 
+// Up to 8-bit digits a pass keeps 4 waves per SIMD (at most 128 VGPRs): the
+// two-input first pass would otherwise take a few registers more and lose
+// one of them.
+template <int BITS, int IN, bool POUT, int T>
+__global__ __launch_bounds__(T, BITS <= 8 ? 4 : 1) void k_downsweep(const uint32_t* __restrict__ keys_in,
+                                                 const int32_t* __restrict__ vals_in, const uint2* __restrict__ pairs_in,
+                                                 int id_mode, int32_t N, uint32_t* __restrict__ keys_out,
+                                                 int32_t* __restrict__ vals_out, uint2* __restrict__ pairs_out,
+                                                 const uint32_t* __restrict__ hist, int64_t n_host,
+                                                 const uint32_t* __restrict__ n_dev, int shift, int nblk,
+                                                 const int32_t* done, uint32_t drop, uint32_t* __restrict__ n_out,
+                                                 SegLayout seg, AltInput alt) {
+    if (is_done(done)) return;
+    __shared__ DownSmem<BITS, T> sm;
+    if constexpr (IN == IN_DUAL) {
+        if (alt_today(alt))
+            downsweep_body<BITS, IN_KEYS_VALS, POUT, T>(sm, alt.keys, alt.vals, nullptr, id_mode, N, keys_out,
+                                                        vals_out, pairs_out, hist, n_host, alt.n_dev, shift, nblk,
+                                                        kNoDrop, n_out, seg);
+        else
+            downsweep_body<BITS, IN_KEYS, POUT, T>(sm, keys_in, nullptr, nullptr, id_mode, N, keys_out, vals_out,
+                                                   pairs_out, hist, n_host, n_dev, shift, nblk, drop, n_out, seg);
+    } else {
+        downsweep_body<BITS, IN, POUT, T>(sm, keys_in, vals_in, pairs_in, id_mode, N, keys_out, vals_out, pairs_out,
+                                          hist, n_host, n_dev, shift, nblk, drop, n_out, seg);
+    }
+}
+
 // Scan, phase 1: per-segment sums.
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
                                                               uint32_t* __restrict__ part, const int32_t* done) {
@@ -545,6 +594,7 @@ struct PassIO {
     uint32_t* kout;
     int32_t* vout;
     uint2* pout;
+    AltInput alt;  // first pass only: the input the device may pick instead of kin
 };
 
 template <int BITS, int T>
@@ -557,24 +607,28 @@ hipError_t radix_pass(TableScratch& s, const PassIO& io, int id_mode, int32_t N,
     const bool grouped = s.grouped_keys && !seg.blk_seg;  // segment-local keys do not come in long runs
     if (pin && grouped)
         launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, true, true, T>, dim3(nblk), dim3(T), st, io.kin, io.pin, n, n_dev,
-               shift, s.hist, nblk, done, drop, seg);
+               shift, s.hist, nblk, done, drop, seg, io.alt);
     else if (pin)
         launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, false, true, T>, dim3(nblk), dim3(T), st, io.kin, io.pin, n, n_dev,
-               shift, s.hist, nblk, done, drop, seg);
+               shift, s.hist, nblk, done, drop, seg, io.alt);
     else if (grouped)
         launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, true, false, T>, dim3(nblk), dim3(T), st, io.kin, io.pin, n, n_dev,
-               shift, s.hist, nblk, done, drop, seg);
+               shift, s.hist, nblk, done, drop, seg, io.alt);
     else
         launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, false, false, T>, dim3(nblk), dim3(T), st, io.kin, io.pin, n,
-               n_dev, shift, s.hist, nblk, done, drop, seg);
+               n_dev, shift, s.hist, nblk, done, drop, seg, io.alt);
     hipError_t e =
         exclusive_scan_u32(s.hist, s.hist, ((int64_t)nblk << BITS) + 1, s.scan_part, done, st, tm, KC_SORT_SCAN);
     if (e != hipSuccess) return e;
     const uint32_t* h = s.hist;
 #define PT_DOWN(PI, PO)                                                                                              \
     launch(tm, KC_SORT_DOWNSWEEP, k_downsweep<BITS, PI, PO, T>, dim3(nblk), dim3(T), st, io.kin, io.vin, io.pin,      \
-           id_mode, N, io.kout, io.vout, io.pout, h, n, n_dev, shift, nblk, done, drop, n_out, seg)
-    if (pin && io.pout)
+           id_mode, N, io.kout, io.vout, io.pout, h, n, n_dev, shift, nblk, done, drop, n_out, seg, io.alt)
+    if (io.alt.flag && io.pout)
+        PT_DOWN(IN_DUAL, true);
+    else if (io.alt.flag)
+        PT_DOWN(IN_DUAL, false);
+    else if (pin && io.pout)
         PT_DOWN(IN_PAIRS, true);
     else if (pin)
         PT_DOWN(IN_PAIRS, false);
@@ -671,10 +725,13 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint
 
 hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int64_t n, const uint32_t* n_dev,
                        int32_t K, IdMode mode, int32_t N, int32_t* people_out, int32_t* offsets_out,
-                       const int32_t* done, hipStream_t stream, LaunchTimer* tm, uint32_t drop_key) {
+                       const int32_t* done, hipStream_t stream, LaunchTimer* tm, uint32_t drop_key,
+                       const AltInput* alt) {
     if (n > s.n_cap || K < 1 || n >= (int64_t)1 << 32) return hipErrorInvalidValue;
     if (drop_key != kNoDrop && (n_dev || drop_key < (uint32_t)K - 1u || drop_key >= (uint32_t)K))
         return hipErrorInvalidValue;  // only the last bucket of a host-sized table can be dropped
+    if (alt && (!alt->flag || !alt->keys || !alt->vals || !alt->n_dev || vals || n_dev))
+        return hipErrorInvalidValue;  // the alternative is explicit ids counted on the device; the main input is not
     const int bits = key_bits(K);
     const int passes = radix_passes(K, s.max_bits);
     // Split the bits as evenly as possible over the passes.
@@ -689,6 +746,7 @@ hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int
             if (p == 0) {
                 io.kin = keys;
                 io.vin = vals;
+                if (alt) io.alt = *alt;
             } else {
                 io.pin = (p & 1) ? s.pairs_a : s.pairs_b;
             }
@@ -699,12 +757,14 @@ hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int
                 io.pout = (p & 1) ? s.pairs_b : s.pairs_a;
             }
             const bool dropping = p == 0 && drop_key != kNoDrop;
+            const bool counting = p == 0 && (dropping || alt);
             hipError_t e = radix_pass_bits(pass_bits[p], s, io, mode, N, n, n_dev, shift, done, stream, tm,
-                                           dropping ? drop_key : kNoDrop, dropping ? s.n_kept : nullptr, SegLayout{});
+                                           dropping ? drop_key : kNoDrop, counting ? s.n_kept : nullptr, SegLayout{});
             if (e != hipSuccess) return e;
-            // After a first pass that dropped keys the remaining passes
-            // sort the pairs it kept (count on the device).
-            if (dropping) n_dev = s.n_kept;
+            // After a first pass that dropped keys (or may have read the
+            // alternative input) the remaining passes sort the pairs it
+            // kept (count on the device).
+            if (counting) n_dev = s.n_kept;
             shift += pass_bits[p];
         }
     }

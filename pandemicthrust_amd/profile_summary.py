@@ -7,9 +7,10 @@ PROFILE_DIR holds the csv output of three runs of the same command:
 `-o trace` (--kernel-trace --stats), `-o fetch` (--pmc FETCH_SIZE) and
 `-o write` (--pmc WRITE_SIZE).  Dispatches are labelled with the classes of
 pt_sim_kernel_name by walking each run's dispatch order: kernels before the
-first sweep are setup; within a day, radix-sort kernels between the errand-key
-generator and the selection kernel build the full table, those after the
-selection kernel build the subset table.  The counters are calibrated with the
+first sweep are setup; within a day, the scans between the selection kernel
+and the run compaction belong to the selection, and the radix-sort kernels
+after the compaction build the day's table (full or subset: the same
+launches).  The counters are calibrated with the
 snapshot-restore copy kernel (k_copy16), which reads and writes COPY_BYTES
 bytes per launch.
 """
@@ -55,15 +56,13 @@ def label(seq):
         elif base in ("k_set_nsel", "k_compact_runs"):
             cls = "select"
             if base == "k_compact_runs":
-                section = "subset"
+                section = "table"
         elif base in SORT:
-            if section == "full":
-                cls = {"k_upsweep": "sort_upsweep", "k_downsweep": "sort_downsweep", "k_offsets": "offsets"}.get(
-                    base, "sort_scan")
-            elif section == "select":
+            if section == "select":
                 cls = "select"
             else:
-                cls = "subset_sort"
+                cls = {"k_upsweep": "sort_upsweep", "k_downsweep": "sort_downsweep", "k_offsets": "offsets"}.get(
+                    base, "sort_scan")
         elif base == "k_contacts":
             cls = "contacts_weekend" if "<true>" in n else "contacts_weekday"
         elif base == "k_apply":
