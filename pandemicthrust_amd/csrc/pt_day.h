@@ -928,8 +928,10 @@ namespace {
 __global__ void k_compact_runs(const int32_t* __restrict__ skip, const uint32_t* __restrict__ stage_keys,
                                const int32_t* __restrict__ stage_ids, const uint32_t* __restrict__ blk_cnt,
                                const uint32_t* __restrict__ blk_off, uint32_t* __restrict__ sel_keys,
-                               int32_t* __restrict__ sel_ids, int stage_stride) {
+                               int32_t* __restrict__ sel_ids, int stage_stride, uint32_t* __restrict__ n_sel,
+                               int nblk) {
     if (*skip) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *n_sel = blk_off[nblk - 1] + blk_cnt[nblk - 1];
     const uint32_t n = blk_cnt[blockIdx.x];
     const uint32_t off = blk_off[blockIdx.x];
     const int64_t src = (int64_t)blockIdx.x * stage_stride;

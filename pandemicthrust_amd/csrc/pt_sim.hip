@@ -283,14 +283,18 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(DayState* __restrict__ st
     }
 }
 
+// Also clears the bucket marks when today marks (k_plan decided).
 __global__ void k_compact(const DayState* __restrict__ st, const int32_t* __restrict__ staging,
                           const uint32_t* __restrict__ blk_ninf, const uint32_t* __restrict__ blk_off,
-                          int32_t* __restrict__ infected) {
+                          int32_t* __restrict__ infected, uint32_t* __restrict__ mark, int64_t nwords) {
     if (st->done) return;
     const uint32_t n = blk_ninf[blockIdx.x];
     const uint32_t off = blk_off[blockIdx.x];
     const int32_t* src = staging + (int64_t)blockIdx.x * kSweepTile;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) infected[off + i] = src[i];
+    if (st->skip_mark) return;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += stride) mark[i] = 0u;
 }
 
 // D3w: kernel_assignWeekdayAfterschoolAndErrands (PandemicSim.cu:2972-3022).
@@ -352,12 +356,6 @@ __global__ __launch_bounds__(256) void k_gen_weekend(const Model* __restrict__ M
 
 // ---------------- subset errand tables ----------------
 
-__global__ void k_clear_marks(const DayState* __restrict__ st, uint32_t* __restrict__ mark, int64_t nwords) {
-    if (st->done || st->skip_mark) return;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += stride) mark[i] = 0u;
-}
-
 // Chooses today's table path from the number of queried buckets.
 __global__ __launch_bounds__(kPlanThreads) void k_decide(DayState* __restrict__ st, const uint32_t* __restrict__ mark,
                                                          int64_t nwords, int32_t K, int forced, int share_pct) {
@@ -404,12 +402,6 @@ __global__ __launch_bounds__(kSelThreads) void k_select_weekend(const Model* __r
     if (!select_block<kSelPerWe, true>(M, st->errand_off, nullptr, mark, 0u, false, b * kSelAgentsWe, M.N,
                                        stage_keys + b * kSelStageWe, stage_ids + b * kSelStageWe, blk_cnt + b))
         st->fault = 1;
-}
-
-__global__ void k_set_nsel(DayState* __restrict__ st, const uint32_t* __restrict__ blk_cnt,
-                           const uint32_t* __restrict__ blk_off, int nblk) {
-    if (st->skip_subset) return;
-    st->n_sel = blk_off[nblk - 1] + blk_cnt[nblk - 1];
 }
 
 // Marks the errand buckets today's infectors draw contacts from.
@@ -959,18 +951,18 @@ int enqueue_day(pt_sim* s, int d) {
     const int32_t lim_we = (int32_t)std::min<int64_t>(mark_limit((int64_t)kWeHours * s->model.we_E, mpct), INT32_MAX);
     launch(tm, KC_PLAN, k_plan, dim3(1), dim3(kPlanThreads), stc, st, s->nblk_sweep, (const uint32_t*)s->blk_counts,
            (const uint32_t*)s->blk_ninf, s->blk_off, s->d_counts, s->d_inf, s->N, lim_wd, lim_we, s->table_mode);
-    launch(tm, KC_COMPACT, k_compact, dim3(s->nblk_sweep), dim3(256), stc, (const DayState*)st,
-           (const int32_t*)s->staging, (const uint32_t*)s->blk_ninf, (const uint32_t*)s->blk_off, s->infected);
     const int64_t n = weekend ? 3 * s->N : 2 * s->N;
     const int32_t K = weekend ? kWeHours * s->model.we_E : s->model.wd_K;
     const IdMode mode = weekend ? ID_DIV3 : ID_HOUR_MAJOR;
+    // Which buckets will be queried today (marks cleared by the compaction),
+    // and which table path to take.
+    const int64_t nwords = ((int64_t)K + 31) / 32;
+    launch(tm, KC_COMPACT, k_compact, dim3(s->nblk_sweep), dim3(256), stc, (const DayState*)st,
+           (const int32_t*)s->staging, (const uint32_t*)s->blk_ninf, (const uint32_t*)s->blk_off, s->infected,
+           s->mark, nwords);
     AgentArrays A = agent_arrays(s);
     const Model* M = s->d_model;
     const int32_t* infected = s->infected;
-    // Which buckets will be queried today, and which table path to take.
-    const int64_t nwords = ((int64_t)K + 31) / 32;
-    launch(tm, KC_MARK, k_clear_marks, dim3(grid_for(nwords, 256, 1024)), dim3(256), stc, (const DayState*)st, s->mark,
-           nwords);
     if (weekend)
         launch(tm, KC_MARK, k_mark_weekend, dim3(s->grid_contacts), dim3(256), stc, M, (const DayState*)st, infected, A,
                s->mark);
@@ -996,11 +988,9 @@ int enqueue_day(pt_sim* s, int d) {
     hipError_t e =
         exclusive_scan_u32(s->sblk_cnt, s->sblk_off, nb, s->tables.scan_part, &st->skip_subset, stc, tm, KC_SELECT);
     if (e != hipSuccess) return fail(PT_EDEVICE, "selection scan: %s", hipGetErrorString(e));
-    launch(tm, KC_SELECT, k_set_nsel, dim3(1), dim3(1), stc, st, (const uint32_t*)s->sblk_cnt,
-           (const uint32_t*)s->sblk_off, nb);
     launch(tm, KC_SELECT, k_compact_runs, dim3(nb), dim3(256), stc, (const int32_t*)&st->skip_subset,
            (const uint32_t*)s->stage_keys, (const int32_t*)s->stage_ids, (const uint32_t*)s->sblk_cnt,
-           (const uint32_t*)s->sblk_off, s->sel_keys, s->sel_ids, weekend ? kSelStageWe : kSelStageWd);
+           (const uint32_t*)s->sblk_off, s->sel_keys, s->sel_ids, weekend ? kSelStageWe : kSelStageWd, &st->n_sel, nb);
     // One table build serves both paths: its first pass reads the selected
     // pairs on subset days (skip_full set) and every agent's keys otherwise.
     // Weekday: the last bucket collects the hour-1 keys nobody queries

@@ -9,8 +9,10 @@
 // Data between passes: the first pass reads the caller's keys (ids explicit
 // or implied by the position) and writes interleaved (key, id) pairs, one
 // 8-byte store per pair; middle passes read and write pairs; the last pass
-// writes keys and ids to separate arrays (ids are the table, keys feed the
-// bucket offsets).  Passes of at least TableScratch::wide_bits bits (8 by
+// writes the ids (the table) and, beside them, each key's bits below the
+// last digit in 1 or 2 bytes: a bucket's offset is a binary search for its
+// low bits inside its last digit's range, which the pass's scanned
+// histogram gives (segmented builds keep whole local keys).  Passes of at least TableScratch::wide_bits bits (8 by
 // default) run 1024-thread blocks on 8192-pair tiles (digit runs of ~32
 // pairs), the others 512-thread blocks on 4096-pair tiles.
 #include "pt_sort.h"
@@ -61,6 +63,11 @@ __device__ __forceinline__ bool is_done(const int32_t* done) { return done && *d
 // with implied ids unless the device flag picks the other keys with
 // explicit ids.
 enum DownIn : int { IN_PAIRS = 0, IN_KEYS = 1, IN_KEYS_VALS = 2, IN_DUAL = 3 };
+
+// Downsweep output: interleaved pairs (passes before the last), or the ids
+// plus the low 8 / 16 / 32 bits of each key (byte, short or word array), or
+// the ids alone.
+enum DownOut : int { OUT_PAIRS = 0, OUT_IDS = 1, OUT_K8 = 2, OUT_K16 = 3, OUT_K32 = 4 };
 
 // The first pass of a table with an AltInput reads the other input today.
 __device__ __forceinline__ bool alt_today(const AltInput& a) { return a.flag && *a.flag; }
@@ -233,8 +240,8 @@ __global__ __launch_bounds__(T) void k_upsweep(const uint32_t* __restrict__ keys
 // digit, while the next tile's pairs are already in flight.  Input (IN):
 // keys with ids implied by the position, keys with explicit ids, or pairs;
 // each is its own instantiation, so the unrolled loads are branch-free and
-// stay in flight; output:
-// pairs (POUT true) or separate keys and ids.
+// stay in flight.  Output (OUT): pairs, or ids with the keys' low 8, 16 or
+// 32 bits beside them, or ids alone.
 template <int BITS, int T>
 struct DownSmem {
     uint2 pairs[T * kItems];
@@ -247,7 +254,7 @@ struct DownSmem {
     uint32_t kept;
 };
 
-template <int BITS, int IN, bool POUT, int T>
+template <int BITS, int IN, int OUT, int T>
 __device__ __forceinline__ void downsweep_body(DownSmem<BITS, T>& sm, const uint32_t* __restrict__ keys_in,
                                                const int32_t* __restrict__ vals_in,
                                                const uint2* __restrict__ pairs_in, int id_mode, int32_t N,
@@ -407,11 +414,16 @@ __device__ __forceinline__ void downsweep_body(DownSmem<BITS, T>& sm, const uint
 #pragma unroll
             for (int j = 0; j < kItems; ++j) {
                 if (tid + j * T < kept_n) {
-                    if (POUT) {
+                    if constexpr (OUT == OUT_PAIRS) {
                         pairs_out[gg[j]] = pp[j];
                     } else {
-                        keys_out[gg[j]] = pp[j].x;
                         vals_out[gg[j]] = (int32_t)pp[j].y;
+                        if constexpr (OUT == OUT_K8)
+                            reinterpret_cast<uint8_t*>(keys_out)[gg[j]] = (uint8_t)pp[j].x;
+                        else if constexpr (OUT == OUT_K16)
+                            reinterpret_cast<uint16_t*>(keys_out)[gg[j]] = (uint16_t)pp[j].x;
+                        else if constexpr (OUT == OUT_K32)
+                            keys_out[gg[j]] = pp[j].x;
                     }
                 }
             }
@@ -429,7 +441,7 @@ __device__ __forceinline__ void downsweep_body(DownSmem<BITS, T>& sm, const uint
 // Up to 8-bit digits a pass keeps 4 waves per SIMD (at most 128 VGPRs): the
 // two-input first pass would otherwise take a few registers more and lose
 // one of them.
-template <int BITS, int IN, bool POUT, int T>
+template <int BITS, int IN, int OUT, int T>
 __global__ __launch_bounds__(T, BITS <= 8 ? 4 : 1) void k_downsweep(const uint32_t* __restrict__ keys_in,
                                                  const int32_t* __restrict__ vals_in, const uint2* __restrict__ pairs_in,
                                                  int id_mode, int32_t N, uint32_t* __restrict__ keys_out,
@@ -442,14 +454,14 @@ __global__ __launch_bounds__(T, BITS <= 8 ? 4 : 1) void k_downsweep(const uint32
     __shared__ DownSmem<BITS, T> sm;
     if constexpr (IN == IN_DUAL) {
         if (alt_today(alt))
-            downsweep_body<BITS, IN_KEYS_VALS, POUT, T>(sm, alt.keys, alt.vals, nullptr, id_mode, N, keys_out,
+            downsweep_body<BITS, IN_KEYS_VALS, OUT, T>(sm, alt.keys, alt.vals, nullptr, id_mode, N, keys_out,
                                                         vals_out, pairs_out, hist, n_host, alt.n_dev, shift, nblk,
                                                         kNoDrop, n_out, seg);
         else
-            downsweep_body<BITS, IN_KEYS, POUT, T>(sm, keys_in, nullptr, nullptr, id_mode, N, keys_out, vals_out,
+            downsweep_body<BITS, IN_KEYS, OUT, T>(sm, keys_in, nullptr, nullptr, id_mode, N, keys_out, vals_out,
                                                    pairs_out, hist, n_host, n_dev, shift, nblk, drop, n_out, seg);
     } else {
-        downsweep_body<BITS, IN, POUT, T>(sm, keys_in, vals_in, pairs_in, id_mode, N, keys_out, vals_out, pairs_out,
+        downsweep_body<BITS, IN, OUT, T>(sm, keys_in, vals_in, pairs_in, id_mode, N, keys_out, vals_out, pairs_out,
                                           hist, n_host, n_dev, shift, nblk, drop, n_out, seg);
     }
 }
@@ -529,23 +541,55 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t* __re
     }
 }
 
-// offsets[q] = first position whose key is >= q (thrust::lower_bound),
-// one binary search per bucket q in [0, K].
-__global__ void k_offsets(const uint32_t* __restrict__ keys, int64_t n_host, const uint32_t* __restrict__ n_dev,
-                          int32_t K, int32_t* __restrict__ offsets, const int32_t* done) {
+// offsets[q] = first position whose key is >= q (thrust::lower_bound), for
+// q in [0, K], after a last pass over digit bits [low_bits, low_bits +
+// top_bits).  hist is that pass's scanned histogram (digit-major, nblk
+// blocks), so the keys with top digit d = q >> low_bits occupy [hist[d *
+// nblk], hist[(d + 1) * nblk]) (the total past the last digit), sorted by
+// their low bits; low holds each pair's key truncated to KW bytes (KW * 8 >=
+// low_bits, so the bits it keeps above low_bits are d's, equal within the
+// range), searched for q truncated alike.  KW = 0: the digit is the key.
+template <int KW>
+__device__ __forceinline__ uint32_t low_key(const void* low, int64_t i) {
+    if constexpr (KW == 1) return reinterpret_cast<const uint8_t*>(low)[i];
+    else if constexpr (KW == 2) return reinterpret_cast<const uint16_t*>(low)[i];
+    else return reinterpret_cast<const uint32_t*>(low)[i];
+}
+
+// Offsets of a table without pairs: every bucket empty.
+__global__ void k_fill_offsets(int32_t K, int32_t* __restrict__ offsets, const int32_t* done) {
     if (is_done(done)) return;
-    const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q <= K; q += stride) offsets[q] = 0;
+}
+
+template <int KW>
+__global__ void k_offsets(const void* __restrict__ low, const uint32_t* __restrict__ hist, int nblk, int top_bits,
+                          int low_bits, int32_t K, int32_t* __restrict__ offsets, const int32_t* done) {
+    if (is_done(done)) return;
+    const int64_t radix = (int64_t)1 << top_bits;
+    const int64_t total = hist[(int64_t)nblk << top_bits];
+    const uint32_t mask = KW == 1 ? 0xFFu : (KW == 2 ? 0xFFFFu : 0xFFFFFFFFu);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q <= K; q += stride) {
-        int64_t lo = 0, hi = n;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if ((int64_t)keys[mid] < q)
-                lo = mid + 1;
-            else
-                hi = mid;
+        const int64_t d = q >> low_bits;
+        if (q == K || d >= radix) {
+            offsets[q] = (int32_t)total;
+            continue;
         }
-        offsets[q] = (int32_t)lo;
+        int64_t a = hist[d * nblk];
+        if constexpr (KW > 0) {
+            int64_t b = d + 1 < radix ? (int64_t)hist[(d + 1) * nblk] : total;
+            const uint32_t t = (uint32_t)q & mask;
+            while (a < b) {
+                const int64_t mid = (a + b) >> 1;
+                if (low_key<KW>(low, mid) < t)
+                    a = mid + 1;
+                else
+                    b = mid;
+            }
+        }
+        offsets[q] = (int32_t)a;
     }
 }
 
@@ -591,10 +635,12 @@ struct PassIO {
     const uint32_t* kin;
     const int32_t* vin;
     const uint2* pin;
-    uint32_t* kout;
+    uint32_t* kout;  // last pass: the keys' low kw bytes (kw 1, 2 or 4; 0: none)
     int32_t* vout;
     uint2* pout;
-    AltInput alt;  // first pass only: the input the device may pick instead of kin
+    int kw;
+    AltInput alt;    // first pass only: the input the device may pick instead of kin
+    int* nblk_out;   // set to the pass's block count when not null
 };
 
 template <int BITS, int T>
@@ -603,7 +649,10 @@ hipError_t radix_pass(TableScratch& s, const PassIO& io, int id_mode, int32_t N,
                       uint32_t* n_out, const SegLayout& seg) {
     const int64_t tiles = (n + (int64_t)T * kItems - 1) / ((int64_t)T * kItems);
     const int nblk = seg.blk_seg ? kSegBlocks : (int)std::min<int64_t>(std::max<int64_t>(tiles, 1), max_blocks<T>());
+    if (io.nblk_out) *io.nblk_out = nblk;
     const bool pin = io.pin != nullptr;
+    if (!io.pout && ((pin && io.kw == 0) || (!pin && io.kw != 0 && (io.kw != 4 || !io.vin || io.alt.flag))))
+        return hipErrorInvalidValue;  // the output forms below are the only ones instantiated
     const bool grouped = s.grouped_keys && !seg.blk_seg;  // segment-local keys do not come in long runs
     if (pin && grouped)
         launch(tm, KC_SORT_UPSWEEP, k_upsweep<BITS, true, true, T>, dim3(nblk), dim3(T), st, io.kin, io.pin, n, n_dev,
@@ -624,22 +673,32 @@ hipError_t radix_pass(TableScratch& s, const PassIO& io, int id_mode, int32_t N,
 #define PT_DOWN(PI, PO)                                                                                              \
     launch(tm, KC_SORT_DOWNSWEEP, k_downsweep<BITS, PI, PO, T>, dim3(nblk), dim3(T), st, io.kin, io.vin, io.pin,      \
            id_mode, N, io.kout, io.vout, io.pout, h, n, n_dev, shift, nblk, done, drop, n_out, seg, io.alt)
-    if (io.alt.flag && io.pout)
-        PT_DOWN(IN_DUAL, true);
-    else if (io.alt.flag)
-        PT_DOWN(IN_DUAL, false);
-    else if (pin && io.pout)
-        PT_DOWN(IN_PAIRS, true);
-    else if (pin)
-        PT_DOWN(IN_PAIRS, false);
-    else if (io.vin && io.pout)
-        PT_DOWN(IN_KEYS_VALS, true);
-    else if (io.vin)
-        PT_DOWN(IN_KEYS_VALS, false);
-    else if (io.pout)
-        PT_DOWN(IN_KEYS, true);
-    else
-        PT_DOWN(IN_KEYS, false);
+    if (io.pout) {
+        if (io.alt.flag)
+            PT_DOWN(IN_DUAL, OUT_PAIRS);
+        else if (pin)
+            PT_DOWN(IN_PAIRS, OUT_PAIRS);
+        else if (io.vin)
+            PT_DOWN(IN_KEYS_VALS, OUT_PAIRS);
+        else
+            PT_DOWN(IN_KEYS, OUT_PAIRS);
+    } else if (pin) {  // the last of several passes
+        if (io.kw == 1)
+            PT_DOWN(IN_PAIRS, OUT_K8);
+        else if (io.kw == 2)
+            PT_DOWN(IN_PAIRS, OUT_K16);
+        else
+            PT_DOWN(IN_PAIRS, OUT_K32);
+    } else if (io.kw == 4) {  // one-pass segmented tables keep their local keys
+        PT_DOWN(IN_KEYS_VALS, OUT_K32);
+    } else {  // one-pass tables: the digit is the key
+        if (io.alt.flag)
+            PT_DOWN(IN_DUAL, OUT_IDS);
+        else if (io.vin)
+            PT_DOWN(IN_KEYS_VALS, OUT_IDS);
+        else
+            PT_DOWN(IN_KEYS, OUT_IDS);
+    }
 #undef PT_DOWN
     return hipGetLastError();
 }
@@ -737,7 +796,18 @@ hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int
     // Split the bits as evenly as possible over the passes.
     int pass_bits[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int p = 0; p < passes; ++p) pass_bits[p] = bits / passes + (p < bits % passes ? 1 : 0);
-    if (n > 0) {
+    const int oblocks = (int)std::min<int64_t>(((int64_t)K + 512) / 512, 4096);
+    if (n == 0) {
+        launch(tm, KC_OFFSETS, k_fill_offsets, dim3(oblocks), dim3(512), stream, K, offsets_out, done);
+        return hipGetLastError();
+    }
+    // The last pass writes the ids and, beside them, each key's bits below
+    // its digit (low_bits) in the narrowest of 1, 2 or 4 bytes; the offsets
+    // search those inside the digit's range.
+    const int low_bits = bits - pass_bits[passes - 1];
+    const int kw = low_bits == 0 ? 0 : (low_bits <= 8 ? 1 : (low_bits <= 16 ? 2 : 4));
+    int last_nblk = 0;
+    {
         // n is the capacity when n_dev carries the actual count.
         int shift = 0;
         for (int p = 0; p < passes; ++p) {
@@ -753,6 +823,8 @@ hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int
             if (last) {
                 io.kout = s.keys_b;
                 io.vout = people_out;
+                io.kw = kw;
+                io.nblk_out = &last_nblk;
             } else {
                 io.pout = (p & 1) ? s.pairs_b : s.pairs_a;
             }
@@ -768,10 +840,21 @@ hipError_t build_table(TableScratch& s, uint32_t* keys, const int32_t* vals, int
             shift += pass_bits[p];
         }
     }
-    // keys_b now holds the sorted keys.
-    const int oblocks = (int)std::min<int64_t>(((int64_t)K + 512) / 512, 4096);
-    launch(tm, KC_OFFSETS, k_offsets, dim3(oblocks), dim3(512), stream, (const uint32_t*)s.keys_b, n, n_dev, K,
-           offsets_out, done);
+    const int top_bits = pass_bits[passes - 1];
+    const void* low = s.keys_b;
+    const uint32_t* h = s.hist;
+    if (kw == 0)
+        launch(tm, KC_OFFSETS, k_offsets<0>, dim3(oblocks), dim3(512), stream, low, h, last_nblk, top_bits, low_bits,
+               K, offsets_out, done);
+    else if (kw == 1)
+        launch(tm, KC_OFFSETS, k_offsets<1>, dim3(oblocks), dim3(512), stream, low, h, last_nblk, top_bits, low_bits,
+               K, offsets_out, done);
+    else if (kw == 2)
+        launch(tm, KC_OFFSETS, k_offsets<2>, dim3(oblocks), dim3(512), stream, low, h, last_nblk, top_bits, low_bits,
+               K, offsets_out, done);
+    else
+        launch(tm, KC_OFFSETS, k_offsets<4>, dim3(oblocks), dim3(512), stream, low, h, last_nblk, top_bits, low_bits,
+               K, offsets_out, done);
     return hipGetLastError();
 }
 
@@ -795,8 +878,9 @@ hipError_t build_table_segmented(TableScratch& s, uint32_t* keys, const int32_t*
             io.pin = (p & 1) ? s.pairs_a : s.pairs_b;
         }
         if (last) {
-            io.kout = s.keys_b;
+            io.kout = s.keys_b;  // whole local keys for k_offsets_seg
             io.vout = people_out;
+            io.kw = 4;
         } else {
             io.pout = (p & 1) ? s.pairs_b : s.pairs_a;
         }

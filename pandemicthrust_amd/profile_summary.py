@@ -45,7 +45,7 @@ def label(seq):
             cls = "plan"
         elif base == "k_compact":
             cls = "compact"
-        elif base in ("k_clear_marks", "k_mark_weekday", "k_mark_weekend", "k_decide"):
+        elif base in ("k_mark_weekday", "k_mark_weekend", "k_decide"):
             cls = "mark"
         elif base in ("k_gen_weekday", "k_gen_weekend"):
             cls = base[2:]
@@ -53,10 +53,9 @@ def label(seq):
         elif base in ("k_select_weekday", "k_select_weekend"):
             cls = "select"
             section = "select"
-        elif base in ("k_set_nsel", "k_compact_runs"):
+        elif base == "k_compact_runs":
             cls = "select"
-            if base == "k_compact_runs":
-                section = "table"
+            section = "table"
         elif base in SORT:
             if section == "select":
                 cls = "select"
