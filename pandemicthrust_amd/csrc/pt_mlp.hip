@@ -106,42 +106,98 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm(GemmArgs g) {
     const bf16x8 zero8 = (bf16x8)(bf16)0.0f;
     // Global reads are 16-byte vectors of 8 consecutive elements of the
     // contiguous dimension (every stride, offset and extent here is a
-    // multiple of 8, so a vector is wholly inside or wholly outside).
+    // multiple of 8, so a vector is wholly inside or wholly outside).  An
+    // operand stored K-major (A with TA, B without TB) is read by one wave
+    // (wave 0 for A, wave 1 for B) as 8 x 8 blocks, 8 K rows of 8
+    // consecutive M or N, transposed in registers and stored as 16-byte LDS
+    // rows of 8 K values.  The next K step's vectors are loaded before the
+    // current step's products, so their latency overlaps the MFMA work.
     constexpr int kVec = kTM * kTK / 8 / kGemmThreads;  // vectors per thread per operand
-    for (int k0 = 0; k0 < g.K; k0 += kTK) {
+    static_assert(kTM == 64 && kTN == 64 && kTK == 64, "the 8 x 8 block loads assume 64 x 64 tiles");
+    bf16x8 ra[TA ? 8 : kVec], rb[TB ? kVec : 8];
+    const int bk = (lane >> 3) * 8, bmn = (lane & 7) * 8;  // this lane's 8 x 8 block: K rows, M / N columns
+    auto load = [&](int k0) {
+        if constexpr (TA) {
+            if (wave == 0) {
+                const int m = m0 + bmn;
 #pragma unroll
-        for (int q = 0; q < kVec; ++q) {
-            const int v = tid + q * kGemmThreads;
-            if (!TA) {
+                for (int r = 0; r < 8; ++r) {
+                    const int k = k0 + bk + r;
+                    ra[r] = zero8;
+                    if (m < g.M && k < g.K) ra[r] = *reinterpret_cast<const bf16x8*>(A + (int64_t)k * g.lda + m);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < kVec; ++q) {
+                const int v = tid + q * kGemmThreads;
                 const int mm = v / (kTK / 8), kk = (v % (kTK / 8)) * 8;
                 const int m = m0 + mm, k = k0 + kk;
-                bf16x8 x = zero8;
-                if (m < g.M && k < g.K) x = *reinterpret_cast<const bf16x8*>(A + (int64_t)m * g.lda + k);
-                *reinterpret_cast<bf16x8*>(&sA[mm][kk]) = x;
-            } else {
-                const int kk = v / (kTM / 8), mm = (v % (kTM / 8)) * 8;
-                const int m = m0 + mm, k = k0 + kk;
-                bf16x8 x = zero8;
-                if (m < g.M && k < g.K) x = *reinterpret_cast<const bf16x8*>(A + (int64_t)k * g.lda + m);
-#pragma unroll
-                for (int i = 0; i < 8; ++i) sA[mm + i][kk] = x[i];
-            }
-            if (TB) {
-                const int nn = v / (kTK / 8), kk = (v % (kTK / 8)) * 8;
-                const int n = n0 + nn, k = k0 + kk;
-                bf16x8 x = zero8;
-                if (n < g.N && k < g.K) x = *reinterpret_cast<const bf16x8*>(B + (int64_t)n * g.ldb + k);
-                *reinterpret_cast<bf16x8*>(&sB[nn][kk]) = x;
-            } else {
-                const int kk = v / (kTN / 8), nn = (v % (kTN / 8)) * 8;
-                const int n = n0 + nn, k = k0 + kk;
-                bf16x8 x = zero8;
-                if (n < g.N && k < g.K) x = *reinterpret_cast<const bf16x8*>(B + (int64_t)k * g.ldb + n);
-#pragma unroll
-                for (int i = 0; i < 8; ++i) sB[nn + i][kk] = x[i];
+                ra[q] = zero8;
+                if (m < g.M && k < g.K) ra[q] = *reinterpret_cast<const bf16x8*>(A + (int64_t)m * g.lda + k);
             }
         }
+        if constexpr (!TB) {
+            if (wave == 1) {
+                const int n = n0 + bmn;
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const int k = k0 + bk + r;
+                    rb[r] = zero8;
+                    if (n < g.N && k < g.K) rb[r] = *reinterpret_cast<const bf16x8*>(B + (int64_t)k * g.ldb + n);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < kVec; ++q) {
+                const int v = tid + q * kGemmThreads;
+                const int nn = v / (kTK / 8), kk = (v % (kTK / 8)) * 8;
+                const int n = n0 + nn, k = k0 + kk;
+                rb[q] = zero8;
+                if (n < g.N && k < g.K) rb[q] = *reinterpret_cast<const bf16x8*>(B + (int64_t)n * g.ldb + k);
+            }
+        }
+    };
+    // Row c of the transposed block: element r is x[r][c].
+    auto column = [](const bf16x8 (&x)[8], int c) {
+        bf16x8 o;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) o[r] = x[r][c];
+        return o;
+    };
+    auto stage = [&]() {
+        if constexpr (TA) {
+            if (wave == 0) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) *reinterpret_cast<bf16x8*>(&sA[bmn + c][bk]) = column(ra, c);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < kVec; ++q) {
+                const int v = tid + q * kGemmThreads;
+                const int mm = v / (kTK / 8), kk = (v % (kTK / 8)) * 8;
+                *reinterpret_cast<bf16x8*>(&sA[mm][kk]) = ra[q];
+            }
+        }
+        if constexpr (!TB) {
+            if (wave == 1) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) *reinterpret_cast<bf16x8*>(&sB[bmn + c][bk]) = column(rb, c);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < kVec; ++q) {
+                const int v = tid + q * kGemmThreads;
+                const int nn = v / (kTK / 8), kk = (v % (kTK / 8)) * 8;
+                *reinterpret_cast<bf16x8*>(&sB[nn][kk]) = rb[q];
+            }
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < g.K; k0 += kTK) {
+        stage();
         __syncthreads();
+        if (k0 + kTK < g.K) load(k0 + kTK);
 #pragma unroll
         for (int kc = 0; kc < kTK; kc += 32) {
             const int kq = kc + 8 * (lane >> 4);
