@@ -32,9 +32,13 @@ def build(keys, K, entry="pt_location_table"):
     return dp.host(), do.host()
 
 
+# Bucket counts cover every form of the last radix pass's key output: one
+# pass (the digit is the key: K <= 256), and the key bits below the last
+# digit kept in 1 byte (K = 545, 61,597), 2 bytes (276,120 up to 2,718,720)
+# and 4 bytes (2^25 + 3: 26 bits in 4 passes, 20 bits below the last digit).
 CASES = [
-    (0, 1), (1, 1), (5, 3), (4096, 1), (100_000, 545), (1_000_000, 61_597), (3_000_000, 276_120),
-    (2_000_000, 1 << 20), (777_777, 2_718_720),
+    (0, 1), (1, 1), (5, 3), (4096, 1), (1_000_000, 256), (100_000, 545), (1_000_000, 61_597),
+    (3_000_000, 276_120), (2_000_000, 1 << 20), (777_777, 2_718_720), (500_000, (1 << 25) + 3),
 ]
 
 
