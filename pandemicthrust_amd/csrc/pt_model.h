@@ -225,16 +225,16 @@ __device__ __forceinline__ RowPick walk_errand_rows(const Model& M, const float*
 }
 
 // device_fishAfterschoolOrErrandDestination_weekday (PandemicSim.cu:2909-2947)
+// Every lane walks the rows and divides, and the child's after-school pick
+// is selected afterwards: a wave holds adults and children alike, so a
+// branch would run both sides anyway, plus its mask handling.
 __device__ __forceinline__ int32_t fish_weekday_dest(const Model& M, uint32_t r, bool adult) {
-    float frac = unit_float(r);
-    int type_count = M.wp_count[kTypeAfterschool];
-    int type_offset = M.wp_offset[kTypeAfterschool];
-    if (adult) {
-        const RowPick p = walk_errand_rows(M, M.wd_pdf, frac);
-        frac = p.frac / p.pdf;
-        type_count = p.count;
-        type_offset = p.offset;
-    }
+    const float u = unit_float(r);
+    const RowPick p = walk_errand_rows(M, M.wd_pdf, u);
+    const float q = p.frac / p.pdf;
+    const float frac = adult ? q : u;
+    const int type_count = adult ? p.count : M.wp_count[kTypeAfterschool];
+    const int type_offset = adult ? p.offset : M.wp_offset[kTypeAfterschool];
     uint32_t bn = (uint32_t)(frac * (float)type_count);
     if (bn >= (uint32_t)type_count) bn = (uint32_t)(type_count - 1);
     bn += (uint32_t)type_offset;
@@ -256,7 +256,8 @@ __device__ __forceinline__ int32_t fish_weekend_dest(const Model& M, uint32_t r)
 __device__ __forceinline__ void weekday_dests(const Model& M, uint32_t r1, uint32_t r2, bool adult, int32_t& d1,
                                               int32_t& d2) {
     d1 = fish_weekday_dest(M, r1, adult);
-    d2 = adult ? fish_weekday_dest(M, r2, adult) : d1;
+    const int32_t e2 = fish_weekday_dest(M, r2, true);
+    d2 = adult ? e2 : d1;
 }
 
 // ---------------- interventions ----------------
@@ -327,7 +328,10 @@ __device__ __forceinline__ void weekend_errands(const Model& M, uint64_t off, in
     int h2 = (int)(a.w[1] % kWeHours);
     if (h2 == h1) h2 = (h2 + 1) % kWeHours;
     int h3 = (int)(a.w[2] % kWeHours);
-    while (h3 == h1 || h3 == h2) h3 = (h3 + 1) % kWeHours;
+    // while (h3 == h1 || h3 == h2) h3 = (h3 + 1) % kWeHours, unrolled: two
+    // forbidden hours end it within two steps.
+    h3 = (h3 == h1 || h3 == h2) ? (h3 + 1) % kWeHours : h3;
+    h3 = (h3 == h1 || h3 == h2) ? (h3 + 1) % kWeHours : h3;
     hour[0] = h1;
     hour[1] = h2;
     hour[2] = h3;
