@@ -311,11 +311,15 @@ __host__ __device__ __forceinline__ uint64_t vaccination_counter(int64_t i) { re
 // [wd_base1, W) at keys wd_E + (d2 - wd_base1); every hour-1 entry below it
 // goes to the last bucket wd_K - 1, which nobody queries.  Returns false for
 // a destination outside [wd_base, W) (the keys are then clamped in range).
+// (Bitwise & and selects, not && and ?: over loads: the compiler would
+// otherwise branch around the model reads.)
 __device__ __forceinline__ bool weekday_keys(const Model& M, int32_t d1, int32_t d2, uint32_t& k0, uint32_t& k1) {
-    const int32_t l1 = d1 - M.wd_base, l2 = d2 - M.wd_base1;
-    const bool ok = (uint32_t)l1 < (uint32_t)M.wd_E && d2 >= M.wd_base && d2 < M.W;
-    k0 = (uint32_t)min(max(l1, 0), M.wd_E - 1);
-    k1 = l2 >= 0 ? (uint32_t)(M.wd_E + min(l2, M.wd_E1 - 1)) : (uint32_t)(M.wd_K - 1);
+    const int32_t E = M.wd_E, E1 = M.wd_E1, K = M.wd_K, base = M.wd_base, base1 = M.wd_base1, W = M.W;
+    const int32_t l1 = d1 - base, l2 = d2 - base1;
+    const bool ok = ((uint32_t)l1 < (uint32_t)E) & (d2 >= base) & (d2 < W);
+    k0 = (uint32_t)min(max(l1, 0), E - 1);
+    const uint32_t hour1 = (uint32_t)(E + min(l2, E1 - 1));
+    k1 = l2 >= 0 ? hour1 : (uint32_t)(K - 1);
     return ok;
 }
 
