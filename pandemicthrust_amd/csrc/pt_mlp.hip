@@ -82,6 +82,9 @@ struct GemmArgs {
     bf16* out_bf16;     // activations / input gradients
     int64_t bf16_z;
     int64_t ldc;
+    // EPI_GELU_GRAD (may be null): the FP32 column sums of each block's 64
+    // output rows, colpart[z][blockIdx.y][col] (N columns per row tile).
+    float* colpart;
 };
 
 // C[M][N] = A(m, k) * B(k, n): A(m, k) = TA ? A[k lda + m] : A[m lda + k],
@@ -213,6 +216,7 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm(GemmArgs g) {
         }
         __syncthreads();
     }
+    float cs[2] = {0.f, 0.f};  // EPI_GELU_GRAD: this lane's column sums
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
@@ -234,10 +238,29 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm(GemmArgs g) {
                     const float d = v * gelu_grad(g.pre[z * g.pre_z + o]);
                     g.out_f32[z * g.f32_z + o] = d;
                     g.out_bf16[z * g.bf16_z + o] = (bf16)d;
+                    cs[tj] += d;
                 } else {
                     g.out_f32[z * g.f32_z + o] = v;
                 }
             }
+    if constexpr (EPI == EPI_GELU_GRAD) {
+        // Column sums of the tile: this lane's 8 rows, then the 4 lane
+        // groups of a 16-column slice, then the two row halves, in that
+        // fixed order.
+        if (g.colpart) {
+            __shared__ float s_cs[2][kTN];
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj) {
+                cs[tj] += __shfl_xor(cs[tj], 16, 64);
+                cs[tj] += __shfl_xor(cs[tj], 32, 64);
+                if (lane < 16) s_cs[wm / 32][wn + tj * 16 + lane] = cs[tj];
+            }
+            __syncthreads();
+            const int col = n0 + tid;
+            if (tid < kTN && col < g.N)
+                g.colpart[((int64_t)z * gridDim.y + blockIdx.y) * g.N + col] = s_cs[0][tid] + s_cs[1][tid];
+        }
+    }
 }
 
 // Batch gather: xb[b][i] = bf16(X[rows[b]][i]), yb[b][j] = Y[rows[b]][j].
@@ -311,12 +334,12 @@ __global__ void k_loss_final(const float* __restrict__ part, int nblk, int membe
     loss[z * loss_z + (step ? step[0] : 0)] = t / (float)n;
 }
 
-// Bias gradients: db[n] = sum over rows of dz[row][n], per member, in two
-// launches.  k_colsum_part: a block sums a chunk of kColRows rows for 64
-// columns (4 row groups, added in a fixed order) into part[z][chunk][n];
-// k_colsum_final adds the chunks in order.  The grid covers columns x row
-// chunks x members, so the sums spread over the chip, and the result does
-// not depend on scheduling.
+// Bias gradients: db[n] = sum over rows of dz[row][n], per member.  Partial
+// sums per row chunk: for a hidden layer the input-gradient GEMM's epilogue
+// leaves them (64-row tiles); for the output layer k_colsum_part sums a
+// chunk of kColRows rows for 64 columns (4 row groups, added in a fixed
+// order) into part[z][chunk][n].  k_colsum_final adds the chunks in order.
+// Every order is fixed, so the result does not depend on scheduling.
 constexpr int kColCols = 64, kColGroups = 4, kColRows = 128;
 __global__ __launch_bounds__(kColCols * kColGroups) void k_colsum_part(const float* __restrict__ dz, int M, int N,
                                                                       int64_t dz_z, float* __restrict__ part) {
@@ -399,7 +422,7 @@ struct pt_mlp {
     float* dz32[2] = {};
     bf16* dz16[2] = {};
     float* part = nullptr;
-    float* colpart = nullptr;  // bias-gradient partial sums [members][row chunks][kMaxWidth]
+    float* colpart = nullptr;  // bias-gradient partial sums [members][row chunks or tiles][kMaxWidth]
     float* loss = nullptr;
     int64_t loss_cap = 0;
     float* xin = nullptr;  // prediction inputs
@@ -488,10 +511,15 @@ void backward(pt_mlp* m, float* loss, int64_t loss_z, const int64_t* step) {
         g.f32_z = P;
         g.ldc = nout;
         gemm<true, false, EPI_F32>(m, g);
-        // db_l
-        const int R = (B + kColRows - 1) / kColRows;
-        k_colsum_part<<<dim3((nout + kColCols - 1) / kColCols, R, m->members), kColCols * kColGroups, 0,
-                        m->stream>>>(m->dz32[cur], B, nout, dz_z(m), m->colpart);
+        // db_l: the output layer's dz comes from the loss kernel and is summed
+        // here in 128-row chunks; a hidden layer's column sums per 64-row
+        // tile came with its dz from the input-gradient GEMM's epilogue.
+        int R = (B + kTM - 1) / kTM;
+        if (l == PT_MLP_LAYERS - 1) {
+            R = (B + kColRows - 1) / kColRows;
+            k_colsum_part<<<dim3((nout + kColCols - 1) / kColCols, R, m->members), kColCols * kColGroups, 0,
+                            m->stream>>>(m->dz32[cur], B, nout, dz_z(m), m->colpart);
+        }
         k_colsum_final<<<(m->members * nout + 255) / 256, 256, 0, m->stream>>>(m->colpart, R, nout, m->members,
                                                                               m->grads + m->L.b[l], P);
         if (l == 0) break;
@@ -513,6 +541,7 @@ void backward(pt_mlp* m, float* loss, int64_t loss_z, const int64_t* step) {
         h.out_bf16 = m->dz16[1 - cur];
         h.bf16_z = dz_z(m);
         h.ldc = nin;
+        h.colpart = m->colpart;
         gemm<false, true, EPI_GELU_GRAD>(m, h);
         cur = 1 - cur;
     }
@@ -556,7 +585,7 @@ int pt_mlp_create(int32_t members, int32_t batch, int32_t device, pt_mlp** out) 
                dev_alloc(&m->yb, (int64_t)batch * PT_MLP_OUT) ||
                dev_alloc(&m->pred, (int64_t)members * batch * PT_MLP_OUT) ||
                dev_alloc(&m->part, (int64_t)members * kLossBlocks) ||
-               dev_alloc(&m->colpart, (int64_t)members * ((batch + kColRows - 1) / kColRows) * kMaxWidth);
+               dev_alloc(&m->colpart, (int64_t)members * ((batch + kTM - 1) / kTM) * kMaxWidth);
     for (int l = 1; l < PT_MLP_LAYERS && !bad; ++l) bad = dev_alloc(&m->act[l], members * act_z(m)) != 0;
     for (int l = 0; l < PT_MLP_LAYERS - 1 && !bad; ++l) bad = dev_alloc(&m->pre[l], members * act_z(m)) != 0;
     for (int i = 0; i < 2 && !bad; ++i)
