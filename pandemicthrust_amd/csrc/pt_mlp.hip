@@ -375,14 +375,43 @@ __global__ void k_colsum_final(const float* __restrict__ part, int R, int N, int
 // Adam on the FP32 master parameters, then their BF16 copies.
 // Adam with bias corrections 1 - beta^t for t = step[1] + 1 (the Adam
 // steps taken before this one, plus this one).
+// Bias gradients (colpart set): summed here from layer l's partial sums,
+// slice l of colpart ([member][R][width], R = r_out chunks for the output
+// layer, r_hidden tiles for the others), in chunk order, the order
+// k_colsum_final uses, instead of read from g.  Grid: x over one member's
+// count parameters, y = member.
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
-                       float* __restrict__ m2, bf16* __restrict__ pb, int64_t n, float lr,
-                       const int64_t* __restrict__ step) {
+                       float* __restrict__ m2, bf16* __restrict__ pb, int64_t count, float lr,
+                       const int64_t* __restrict__ step, const float* __restrict__ colpart, int64_t slice,
+                       int r_hidden, int r_out) {
     const double t = (double)(step[1] + 1);
     const float bc1 = (float)(1.0 - pow(0.9, t)), bc2 = (float)(1.0 - pow(0.999, t));
+    const int z = blockIdx.y;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const float gi = g[i];
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count; j += stride) {
+        const int64_t i = (int64_t)z * count + j;
+        int bl = -1;
+        int64_t bj = 0, o = 0;
+#pragma unroll
+        for (int l = 0; l < PT_MLP_LAYERS; ++l) {
+            o += (int64_t)kDims[l] * kDims[l + 1];
+            if (j >= o && j < o + kDims[l + 1]) {
+                bl = l;
+                bj = j - o;
+            }
+            o += kDims[l + 1];
+        }
+        float gi;
+        if (colpart && bl >= 0) {
+            const int R = bl == PT_MLP_LAYERS - 1 ? r_out : r_hidden;
+            const int N = kDims[bl + 1];
+            const float* c = colpart + bl * slice + (int64_t)z * R * N + bj;
+            float s = 0.f;
+            for (int r = 0; r < R; ++r) s += c[(int64_t)r * N];
+            gi = s;
+        } else {
+            gi = g[i];
+        }
         const float a = 0.9f * m1[i] + 0.1f * gi;
         const float b = 0.999f * m2[i] + 0.001f * gi * gi;
         m1[i] = a;
@@ -422,7 +451,7 @@ struct pt_mlp {
     float* dz32[2] = {};
     bf16* dz16[2] = {};
     float* part = nullptr;
-    float* colpart = nullptr;  // bias-gradient partial sums [members][row chunks or tiles][kMaxWidth]
+    float* colpart = nullptr;  // bias-gradient partial sums [layers][members][row chunks or tiles][kMaxWidth]
     float* loss = nullptr;
     int64_t loss_cap = 0;
     float* xin = nullptr;  // prediction inputs
@@ -484,9 +513,13 @@ void forward(pt_mlp* m, int rows) {
     }
 }
 
+int64_t colpart_slice(const pt_mlp* m);
+
 // Loss and gradients of every member for the batch in xb / yb.
 // step (may be null): the training step counters (the loss column).
-void backward(pt_mlp* m, float* loss, int64_t loss_z, const int64_t* step) {
+// bias_sums: also add the bias gradients' partial sums into grads (else
+// they stay in colpart's per-layer slices for k_adam).
+void backward(pt_mlp* m, float* loss, int64_t loss_z, const int64_t* step, bool bias_sums) {
     const int64_t P = m->L.count;
     const int B = m->batch;
     k_loss<<<dim3(kLossBlocks, m->members), kLossThreads, 0, m->stream>>>(m->pred, m->yb, B, m->dz32[0], m->dz16[0],
@@ -515,13 +548,15 @@ void backward(pt_mlp* m, float* loss, int64_t loss_z, const int64_t* step) {
         // here in 128-row chunks; a hidden layer's column sums per 64-row
         // tile came with its dz from the input-gradient GEMM's epilogue.
         int R = (B + kTM - 1) / kTM;
+        float* part_l = m->colpart + l * colpart_slice(m);
         if (l == PT_MLP_LAYERS - 1) {
             R = (B + kColRows - 1) / kColRows;
             k_colsum_part<<<dim3((nout + kColCols - 1) / kColCols, R, m->members), kColCols * kColGroups, 0,
-                            m->stream>>>(m->dz32[cur], B, nout, dz_z(m), m->colpart);
+                            m->stream>>>(m->dz32[cur], B, nout, dz_z(m), part_l);
         }
-        k_colsum_final<<<(m->members * nout + 255) / 256, 256, 0, m->stream>>>(m->colpart, R, nout, m->members,
-                                                                              m->grads + m->L.b[l], P);
+        if (bias_sums)
+            k_colsum_final<<<(m->members * nout + 255) / 256, 256, 0, m->stream>>>(part_l, R, nout, m->members,
+                                                                                  m->grads + m->L.b[l], P);
         if (l == 0) break;
         // dz_{l-1} = (dz_l W_l^T) * gelu'(pre_{l-1})
         GemmArgs h = args();
@@ -541,11 +576,14 @@ void backward(pt_mlp* m, float* loss, int64_t loss_z, const int64_t* step) {
         h.out_bf16 = m->dz16[1 - cur];
         h.bf16_z = dz_z(m);
         h.ldc = nin;
-        h.colpart = m->colpart;
+        h.colpart = m->colpart + (l - 1) * colpart_slice(m);
         gemm<false, true, EPI_GELU_GRAD>(m, h);
         cur = 1 - cur;
     }
 }
+
+// Floats per layer slice of colpart: members x (row tiles of kTM) x kMaxWidth.
+int64_t colpart_slice(const pt_mlp* m) { return (int64_t)m->members * ((m->batch + kTM - 1) / kTM) * kMaxWidth; }
 
 int upload_rows(pt_mlp* m, const int32_t* rows, int64_t n) {
     for (int64_t i = 0; i < n; ++i)
@@ -585,7 +623,7 @@ int pt_mlp_create(int32_t members, int32_t batch, int32_t device, pt_mlp** out) 
                dev_alloc(&m->yb, (int64_t)batch * PT_MLP_OUT) ||
                dev_alloc(&m->pred, (int64_t)members * batch * PT_MLP_OUT) ||
                dev_alloc(&m->part, (int64_t)members * kLossBlocks) ||
-               dev_alloc(&m->colpart, (int64_t)members * ((batch + kTM - 1) / kTM) * kMaxWidth);
+               dev_alloc(&m->colpart, (int64_t)PT_MLP_LAYERS * members * ((batch + kTM - 1) / kTM) * kMaxWidth);
     for (int l = 1; l < PT_MLP_LAYERS && !bad; ++l) bad = dev_alloc(&m->act[l], members * act_z(m)) != 0;
     for (int l = 0; l < PT_MLP_LAYERS - 1 && !bad; ++l) bad = dev_alloc(&m->pre[l], members * act_z(m)) != 0;
     for (int i = 0; i < 2 && !bad; ++i)
@@ -661,9 +699,10 @@ int pt_mlp_train(pt_mlp* m, const int32_t* order, int32_t steps, float lr, float
         k_gather<<<grid_for((int64_t)B * (PT_MLP_IN + PT_MLP_OUT), 256, 4096), 256, 0, m->stream>>>(
             m->X, m->Y, m->order, m->d_step, B, m->xb, m->yb);
         forward(m, B);
-        backward(m, m->loss, steps, m->d_step);
-        k_adam<<<grid_for(P, 256, 8192), 256, 0, m->stream>>>(m->params, m->grads, m->m1, m->m2, m->wb, P, lr,
-                                                               m->d_step);
+        backward(m, m->loss, steps, m->d_step, false);
+        k_adam<<<dim3(grid_for(m->L.count, 256, 2048), m->members), 256, 0, m->stream>>>(
+            m->params, m->grads, m->m1, m->m2, m->wb, m->L.count, lr, m->d_step, m->colpart, colpart_slice(m),
+            (B + kTM - 1) / kTM, (B + kColRows - 1) / kColRows);
         k_step_done<<<1, 1, 0, m->stream>>>(m->d_step);
     };
     // The instantiated step graph, released on every return.
@@ -728,7 +767,7 @@ int pt_mlp_gradients(pt_mlp* m, const int32_t* rows, float* grads_out, float* lo
     k_gather<<<grid_for((int64_t)B * (PT_MLP_IN + PT_MLP_OUT), 256, 4096), 256, 0, m->stream>>>(
         m->X, m->Y, m->order, (const int64_t*)nullptr, B, m->xb, m->yb);
     forward(m, B);
-    backward(m, m->loss, 1, nullptr);
+    backward(m, m->loss, 1, nullptr, true);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(m->stream));
     HIP_TRY(hipMemcpy(grads_out, m->grads, sizeof(float) * m->L.count * m->members, hipMemcpyDeviceToHost));
