@@ -424,9 +424,22 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 }
 
 // Ends a training step: the next step's index and Adam count.
-__global__ void k_step_done(int64_t* step) {
-    step[0] += 1;
-    step[1] += 1;
+// The end of a training step: k_loss_final for every member (threads
+// 0..members-1, loss column step[0]), then the step index and the Adam
+// step count advance.
+__global__ void k_loss_step(const float* __restrict__ part, int nblk, int members, int64_t n,
+                            float* __restrict__ loss, int64_t loss_z, int64_t* __restrict__ step) {
+    const int z = threadIdx.x;
+    if (z < members) {
+        float t = 0.f;
+        for (int b = 0; b < nblk; ++b) t += part[z * nblk + b];
+        loss[z * loss_z + step[0]] = t / (float)n;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        step[0] += 1;
+        step[1] += 1;
+    }
 }
 
 }  // namespace
@@ -517,15 +530,17 @@ int64_t colpart_slice(const pt_mlp* m);
 
 // Loss and gradients of every member for the batch in xb / yb.
 // step (may be null): the training step counters (the loss column).
-// bias_sums: also add the bias gradients' partial sums into grads (else
-// they stay in colpart's per-layer slices for k_adam).
+// bias_sums: also add the bias gradients' partial sums into grads and the
+// loss partials into loss (else the partials stay for k_adam, in colpart's
+// per-layer slices, and for k_loss_step, in part).
 void backward(pt_mlp* m, float* loss, int64_t loss_z, const int64_t* step, bool bias_sums) {
     const int64_t P = m->L.count;
     const int B = m->batch;
     k_loss<<<dim3(kLossBlocks, m->members), kLossThreads, 0, m->stream>>>(m->pred, m->yb, B, m->dz32[0], m->dz16[0],
                                                                           dz_z(m), m->part);
-    k_loss_final<<<1, 64, 0, m->stream>>>(m->part, kLossBlocks, m->members, (int64_t)B * PT_MLP_OUT, loss, loss_z,
-                                          step);
+    if (bias_sums)
+        k_loss_final<<<1, 64, 0, m->stream>>>(m->part, kLossBlocks, m->members, (int64_t)B * PT_MLP_OUT, loss, loss_z,
+                                              step);
     int cur = 0;
     for (int l = PT_MLP_LAYERS - 1; l >= 0; --l) {
         const int nout = kDims[l + 1], nin = kDims[l];
@@ -703,7 +718,8 @@ int pt_mlp_train(pt_mlp* m, const int32_t* order, int32_t steps, float lr, float
         k_adam<<<dim3(grid_for(m->L.count, 256, 2048), m->members), 256, 0, m->stream>>>(
             m->params, m->grads, m->m1, m->m2, m->wb, m->L.count, lr, m->d_step, m->colpart, colpart_slice(m),
             (B + kTM - 1) / kTM, (B + kColRows - 1) / kColRows);
-        k_step_done<<<1, 1, 0, m->stream>>>(m->d_step);
+        k_loss_step<<<1, 64, 0, m->stream>>>(m->part, kLossBlocks, m->members, (int64_t)B * PT_MLP_OUT, m->loss,
+                                             steps, m->d_step);
     };
     // The instantiated step graph, released on every return.
     struct ExecGuard {
