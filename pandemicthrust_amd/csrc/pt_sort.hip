@@ -12,9 +12,10 @@
 // writes the ids (the table) and, beside them, each key's bits below the
 // last digit in 1 or 2 bytes: a bucket's offset is a binary search for its
 // low bits inside its last digit's range, which the pass's scanned
-// histogram gives (segmented builds keep whole local keys).  Passes of at least TableScratch::wide_bits bits (8 by
-// default) run 1024-thread blocks on 8192-pair tiles (digit runs of ~32
-// pairs), the others 512-thread blocks on 4096-pair tiles.
+// histogram gives (segmented builds keep whole local keys).  Passes of at
+// least TableScratch::wide_bits bits (8 by default) run 1024-thread blocks
+// on 8192-pair tiles (digit runs of ~32 pairs), the others 512-thread
+// blocks on 4096-pair tiles.
 #include "pt_sort.h"
 
 #include <algorithm>
