@@ -220,18 +220,34 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(DayState* __restrict__ st
     __shared__ uint32_t s_red[NW][16];
     __shared__ uint32_t s_w[NW];
     __shared__ uint32_t s_carry;
+    // Counts: each block's 16 counters are one 64-byte row (4 x uint4).
     uint32_t c[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) c[i] = 0;
-    for (int b = tid; b < nblk; b += kPlanThreads)
+    const uint4* rows = reinterpret_cast<const uint4*>(blk_counts);
+    for (int b = tid; b < nblk; b += kPlanThreads) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) c[i] += blk_counts[(int64_t)b * 16 + i];
+        for (int q = 0; q < 4; ++q) {
+            const uint4 v = rows[(int64_t)b * 4 + q];
+            c[4 * q] += v.x;
+            c[4 * q + 1] += v.y;
+            c[4 * q + 2] += v.z;
+            c[4 * q + 3] += v.w;
+        }
+    }
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         const uint32_t v = wave_sum(c[i]);
         if (lane == 0) s_red[wave][i] = v;
     }
-    if (tid == 0) s_carry = 0;
+    // Block offsets: thread tid scans its own run of consecutive blocks, so
+    // one block-wide scan of the run totals suffices.
+    const int per = (nblk + kPlanThreads - 1) / kPlanThreads;
+    const int b0 = tid * per, b1 = min(nblk, b0 + per);
+    uint32_t run = 0;
+    for (int b = b0; b < b1; ++b) run += blk_ninf[b];
+    const uint32_t inc = wave_incl_scan(run, lane);
+    if (lane == 63) s_w[wave] = inc;
     __syncthreads();
     const int day = st->day;
     if (tid < 16) {
@@ -239,23 +255,19 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(DayState* __restrict__ st
         for (int w = 0; w < NW; ++w) s += s_red[w][tid];
         out_counts[day * 16 + tid] = (int32_t)s;
     }
-    for (int b0 = 0; b0 < nblk; b0 += kPlanThreads) {
-        const int b = b0 + tid;
-        const uint32_t v = b < nblk ? blk_ninf[b] : 0u;
-        const uint32_t inc = wave_incl_scan(v, lane);
-        if (lane == 63) s_w[wave] = inc;
-        __syncthreads();
-        uint32_t wbase = 0, tot = 0;
-        for (int w = 0; w < NW; ++w) {
-            if (w < wave) wbase += s_w[w];
-            tot += s_w[w];
-        }
-        const uint32_t carry = s_carry;
-        if (b < nblk) blk_off[b] = carry + wbase + inc - v;
-        __syncthreads();
-        if (tid == 0) s_carry = carry + tot;
-        __syncthreads();
+    uint32_t wbase = 0, tot = 0;
+    for (int w = 0; w < NW; ++w) {
+        if (w < wave) wbase += s_w[w];
+        tot += s_w[w];
     }
+    uint32_t off = wbase + inc - run;
+    for (int b = b0; b < b1; ++b) {
+        const uint32_t v = blk_ninf[b];
+        blk_off[b] = off;
+        off += v;
+    }
+    if (tid == 0) s_carry = tot;
+    __syncthreads();
     if (tid == 0) {
         const int32_t I = (int32_t)s_carry;
         st->n_infected = I;
